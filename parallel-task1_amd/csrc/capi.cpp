@@ -1,0 +1,435 @@
+// capi.cpp -- the extern "C" boundary of libsssp_mi355x.so: handle lifetime,
+// uploads, status codes, and the reference-shaped wrappers.
+#include <cstring>
+#include <string>
+
+#include "mlp_dev.hpp"
+#include "sssp_graph.hpp"
+
+namespace sssp {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int g_fail_next_alloc = 0;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+namespace {
+
+void set_device(int dev)
+{
+    int n = 0;
+    SSSP_HIP(hipGetDeviceCount(&n));
+    SSSP_REQUIRE(dev >= 0 && dev < n, "device index out of range (" + std::to_string(dev) +
+                                          " of " + std::to_string(n) + ")");
+    SSSP_HIP(hipSetDevice(dev));
+}
+
+void graph_free(sssp_graph *g)
+{
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    void *ptrs[] = {g->row_ptr, g->el_src, g->el_dst, g->el_w, g->col_ptr, g->csc_src, g->csc_w,
+                    g->nzo_v, g->nzo_scan, g->nzi_v, g->nzi_scan, g->sample_v, g->dist, g->stamp,
+                    g->queue[0], g->queue[1], g->lb_v, g->lb_start, g->lb_scan, g->lb_tile,
+                    g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch};
+    for (void *p : ptrs) dfree(p);
+    for (int c = 0; c < kChunkCount; c++) {
+        dfree(g->nzo_tile[c]);
+        dfree(g->nzi_tile[c]);
+    }
+    if (g->ctrl_host) (void)hipHostFree(g->ctrl_host);
+    hipEvent_t evs[] = {g->ev_a, g->ev_b, g->ev_r0, g->ev_r1};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+size_t wbytes(int wtype) { return wtype == SSSP_W_F64 ? 8 : 4; }
+
+}  // namespace
+
+}  // namespace sssp
+
+using namespace sssp;
+
+extern "C" {
+
+const char *sssp_last_error(void) { return g_last_error.c_str(); }
+
+int sssp_version(void) { return 10000; }
+
+int sssp_debug_fail_next_alloc(void)
+{
+    g_fail_next_alloc = 1;
+    return SSSP_OK;
+}
+
+int sssp_device_count(int32_t *count)
+{
+    return guard([&] {
+        SSSP_REQUIRE(count, "null count");
+        int n = 0;
+        SSSP_HIP(hipGetDeviceCount(&n));
+        *count = n;
+    });
+}
+
+int sssp_device_info(int32_t device, char *name, int32_t name_len, int32_t *cu_count,
+                     int64_t *hbm_bytes, int32_t *l2_bytes)
+{
+    return guard([&] {
+        set_device(device);
+        hipDeviceProp_t p;
+        SSSP_HIP(hipGetDeviceProperties(&p, device));
+        if (name && name_len > 0) {
+            std::strncpy(name, p.gcnArchName, (size_t)name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (cu_count) *cu_count = p.multiProcessorCount;
+        if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;
+        if (l2_bytes) *l2_bytes = p.l2CacheSize;
+    });
+}
+
+int sssp_graph_create(int32_t weight_type, const int32_t *src, const int32_t *dst,
+                      const void *weights, int32_t vertices, int64_t edges, int32_t device,
+                      uint32_t flags, sssp_graph **out)
+{
+    sssp_graph *g = nullptr;
+    int rc = guard([&] {
+        SSSP_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        SSSP_REQUIRE(weight_type == SSSP_W_F32 || weight_type == SSSP_W_U32,
+                     "graph weight type must be SSSP_W_F32 or SSSP_W_U32");
+        SSSP_REQUIRE(vertices >= 1, "graph needs at least one vertex");
+        SSSP_REQUIRE(edges >= 0 && edges < ((int64_t)1 << 31) - 1,
+                     "edge count must be in [0, 2^31 - 1)");
+        SSSP_REQUIRE(edges == 0 || (src && dst && weights), "null edge array");
+        set_device(device);
+        g = new sssp_graph();
+        g->device = device;
+        g->wtype = weight_type;
+        g->V = vertices;
+        g->E = edges;
+        hipDeviceProp_t p;
+        SSSP_HIP(hipGetDeviceProperties(&p, device));
+        g->cu_count = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+        SSSP_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        hipStream_t s = g->stream;
+
+        // upload (or use the caller's device arrays)
+        const int32_t *d_src = src, *d_dst = dst;
+        const uint32_t *d_w = static_cast<const uint32_t *>(weights);
+        int32_t *u_src = nullptr, *u_dst = nullptr;
+        uint32_t *u_w = nullptr;
+        struct Up {
+            void *p[3];
+            ~Up()
+            {
+                for (void *x : p) dfree(x);
+            }
+        } up{{nullptr, nullptr, nullptr}};
+        if (!(flags & SSSP_INPUT_DEVICE) && edges > 0) {
+            u_src = dmalloc<int32_t>((size_t)edges);
+            up.p[0] = u_src;
+            u_dst = dmalloc<int32_t>((size_t)edges);
+            up.p[1] = u_dst;
+            u_w = dmalloc<uint32_t>((size_t)edges);
+            up.p[2] = u_w;
+            SSSP_HIP(hipMemcpyAsync(u_src, src, 4 * (size_t)edges, hipMemcpyHostToDevice, s));
+            SSSP_HIP(hipMemcpyAsync(u_dst, dst, 4 * (size_t)edges, hipMemcpyHostToDevice, s));
+            SSSP_HIP(hipMemcpyAsync(u_w, weights, 4 * (size_t)edges, hipMemcpyHostToDevice, s));
+            d_src = u_src;
+            d_dst = u_dst;
+            d_w = u_w;
+        }
+        build_layouts(g, d_src, d_dst, d_w, !(flags & SSSP_NO_CSC));
+
+        // solve state
+        const size_t V = (size_t)vertices;
+        g->dist = dmalloc<uint32_t>(V);
+        g->stamp = dmalloc<int32_t>(V);
+        SSSP_HIP(hipMemsetAsync(g->stamp, 0, 4 * V, s));
+        g->queue[0] = dmalloc<int32_t>(V + 1);
+        g->queue[1] = dmalloc<int32_t>(V + 1);
+        g->lb_v = dmalloc<int32_t>(V + 1);
+        g->lb_start = dmalloc<int64_t>(V + 1);
+        g->lb_scan = dmalloc<int64_t>(V + 2);
+        g->lb_tile = dmalloc<int32_t>((size_t)(edges / kBlock) + 4);
+        g->parent = dmalloc<int32_t>(V);
+        g->ctrl = dmalloc<Ctrl>(1);
+        SSSP_HIP(hipMemsetAsync(g->ctrl, 0, sizeof(Ctrl), s));
+        SSSP_HIP(hipHostMalloc((void **)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
+        std::memset(g->ctrl_host, 0, sizeof(Ctrl));
+        // fixed-stride vertex sample for the unsettled-fraction estimate
+        g->n_sample = vertices < 16384 ? vertices : 16384;
+        std::vector<int32_t> sv((size_t)g->n_sample);
+        for (int32_t i = 0; i < g->n_sample; i++)
+            sv[(size_t)i] = (int32_t)(((int64_t)i * vertices) / g->n_sample);
+        g->sample_v = dmalloc<int32_t>((size_t)g->n_sample);
+        SSSP_HIP(hipMemcpyAsync(g->sample_v, sv.data(), 4 * sv.size(), hipMemcpyHostToDevice, s));
+        SSSP_HIP(hipEventCreate(&g->ev_a));
+        SSSP_HIP(hipEventCreate(&g->ev_b));
+        SSSP_HIP(hipEventCreate(&g->ev_r0));
+        SSSP_HIP(hipEventCreate(&g->ev_r1));
+        SSSP_HIP(hipStreamSynchronize(s));
+        const int64_t Ei = edges, Vi = vertices;
+        g->device_bytes = Ei * 12 + (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) +
+                          Vi * 4 * 7 + Vi * 16 + Vi * 16;
+        *out = g;
+    });
+    if (rc != SSSP_OK && g) graph_free(g);
+    return rc;
+}
+
+int sssp_graph_destroy(sssp_graph *g)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g, "null graph");
+        graph_free(g);
+    });
+}
+
+int sssp_graph_info(const sssp_graph *g, int32_t *vertices, int64_t *edges, int32_t *weight_type,
+                    int64_t *device_bytes)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g, "null graph");
+        if (vertices) *vertices = g->V;
+        if (edges) *edges = g->E;
+        if (weight_type) *weight_type = g->wtype;
+        if (device_bytes) *device_bytes = g->device_bytes;
+    });
+}
+
+int sssp_graph_download_layout(const sssp_graph *g, int64_t *row_ptr, int32_t *el_src,
+                               int32_t *el_dst, void *el_w, int64_t *col_ptr, int32_t *csc_src,
+                               void *csc_w)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g, "null graph");
+        set_device(g->device);
+        const size_t V1 = (size_t)g->V + 1, E = (size_t)g->E;
+        hipStream_t s = g->stream;
+        if (row_ptr) SSSP_HIP(hipMemcpyAsync(row_ptr, g->row_ptr, 8 * V1, hipMemcpyDeviceToHost, s));
+        if (el_src) SSSP_HIP(hipMemcpyAsync(el_src, g->el_src, 4 * E, hipMemcpyDeviceToHost, s));
+        if (el_dst) SSSP_HIP(hipMemcpyAsync(el_dst, g->el_dst, 4 * E, hipMemcpyDeviceToHost, s));
+        if (el_w) SSSP_HIP(hipMemcpyAsync(el_w, g->el_w, 4 * E, hipMemcpyDeviceToHost, s));
+        if (col_ptr || csc_src || csc_w) SSSP_REQUIRE(g->has_csc, "graph was built without CSC");
+        if (col_ptr) SSSP_HIP(hipMemcpyAsync(col_ptr, g->col_ptr, 8 * V1, hipMemcpyDeviceToHost, s));
+        if (csc_src) SSSP_HIP(hipMemcpyAsync(csc_src, g->csc_src, 4 * E, hipMemcpyDeviceToHost, s));
+        if (csc_w) SSSP_HIP(hipMemcpyAsync(csc_w, g->csc_w, 4 * E, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int sssp_solve(sssp_graph *g, int32_t source, int32_t schedule, void *dist_out, sssp_stats *stats)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g, "null graph");
+        SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
+        SSSP_REQUIRE(schedule >= SSSP_SCHED_PLAIN && schedule <= SSSP_SCHED_PULL, "unknown schedule");
+        set_device(g->device);
+        sssp_stats st;
+        std::memset(&st, 0, sizeof(st));
+        g->have_result = false;
+        solve(g, source, schedule, &st);
+        g->last_source = source;
+        g->last_sched = schedule;
+        g->have_result = true;
+        if (dist_out)
+            SSSP_HIP(hipMemcpy(dist_out, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+        if (stats) *stats = st;
+    });
+}
+
+int sssp_get_distances(sssp_graph *g, void *dist_out)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && dist_out, "null argument");
+        SSSP_REQUIRE(g->have_result, "no solve result on this graph");
+        set_device(g->device);
+        SSSP_HIP(hipMemcpy(dist_out, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+    });
+}
+
+int sssp_get_trace(sssp_graph *g, int32_t cap, int32_t *modes, int64_t *frontier,
+                   int64_t *frontier_edges, float *round_ms, int32_t *rounds)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && rounds, "null argument");
+        const int32_t n = (int32_t)g->trace.size();
+        *rounds = n;
+        for (int32_t i = 0; i < n && i < cap; i++) {
+            const TraceRow &t = g->trace[(size_t)i];
+            if (modes) modes[i] = t.mode;
+            if (frontier) frontier[i] = t.frontier;
+            if (frontier_edges) frontier_edges[i] = t.frontier_edges;
+            if (round_ms) round_ms[i] = t.ms;
+        }
+    });
+}
+
+int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *counts,
+                 int32_t *valid)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && counts, "null argument");
+        SSSP_REQUIRE(g->have_result, "no solve result on this graph");
+        SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
+        set_device(g->device);
+        certify(g, source, counts);
+        if (parent_out)
+            SSSP_HIP(hipMemcpy(parent_out, g->parent, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+        if (valid) *valid = (counts[0] == 0 && counts[1] == 0 && counts[2] == 0) ? 1 : 0;
+    });
+}
+
+int sssp_set_param(sssp_graph *g, const char *name, double value)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && name, "null argument");
+        const std::string n(name);
+        auto chunk = [&](int &dst) {
+            SSSP_REQUIRE(value >= 0 && value < kChunkCount, "chunk bucket must be in [0, 4)");
+            dst = (int)value;
+        };
+        if (n == "push_chunk") chunk(g->push_chunk);
+        else if (n == "all_chunk") chunk(g->all_chunk);
+        else if (n == "pull_chunk") chunk(g->pull_chunk);
+        else if (n == "advance_blocks") {
+            SSSP_REQUIRE(value >= 0 && value <= 1 << 20, "advance_blocks out of range");
+            g->advance_blocks = (int)value;
+        } else if (n == "verify") g->verify = value != 0;
+        else if (n == "max_rounds") {
+            SSSP_REQUIRE(value >= 1, "max_rounds must be >= 1");
+            g->max_rounds = (int)value;
+        } else if (n == "push_edge_frac") g->push_edge_frac = value;
+        else if (n == "pull_frac") g->pull_frac = value;
+        else if (n == "trace_events") g->trace_events = value != 0;
+        else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};
+    });
+}
+
+int sssp_get_param(const sssp_graph *g, const char *name, double *value)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && name && value, "null argument");
+        const std::string n(name);
+        if (n == "push_chunk") *value = g->push_chunk;
+        else if (n == "all_chunk") *value = g->all_chunk;
+        else if (n == "pull_chunk") *value = g->pull_chunk;
+        else if (n == "advance_blocks") *value = g->advance_blocks;
+        else if (n == "verify") *value = g->verify;
+        else if (n == "max_rounds") *value = g->max_rounds;
+        else if (n == "push_edge_frac") *value = g->push_edge_frac;
+        else if (n == "pull_frac") *value = g->pull_frac;
+        else if (n == "trace_events") *value = g->trace_events;
+        else if (n == "cu_count") *value = g->cu_count;
+        else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};
+    });
+}
+
+// ---- reference-shaped wrappers --------------------------------------------
+int sssp_init_distances_f32(float *d_dist, int32_t vertices, int32_t source)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_dist && vertices >= 1 && source >= 0 && source < vertices, "bad arguments");
+        ref_init<WF32, uint32_t>(reinterpret_cast<uint32_t *>(d_dist), vertices, source);
+    });
+}
+
+int sssp_init_distances_u32(uint32_t *d_dist, int32_t vertices, int32_t source)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_dist && vertices >= 1 && source >= 0 && source < vertices, "bad arguments");
+        ref_init<WU32, uint32_t>(d_dist, vertices, source);
+    });
+}
+
+int sssp_init_distances_f64(double *d_dist, int32_t vertices, int32_t source)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_dist && vertices >= 1 && source >= 0 && source < vertices, "bad arguments");
+        ref_init<WF64, unsigned long long>(reinterpret_cast<unsigned long long *>(d_dist), vertices,
+                                           source);
+    });
+}
+
+int sssp_bellman_ford_f32(const int32_t *d_src, const int32_t *d_dst, const float *d_w,
+                          int32_t vertices, int64_t edges, int32_t source, float *d_dist,
+                          int32_t *passes)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_dist && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst && d_w)),
+                     "bad arguments");
+        SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        ref_bf<WF32, uint32_t>(d_src, d_dst, reinterpret_cast<const uint32_t *>(d_w), vertices, edges,
+                               reinterpret_cast<uint32_t *>(d_dist), passes);
+    });
+}
+
+int sssp_bellman_ford_u32(const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
+                          int32_t vertices, int64_t edges, int32_t source, uint32_t *d_dist,
+                          int32_t *passes)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_dist && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst && d_w)),
+                     "bad arguments");
+        SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        ref_bf<WU32, uint32_t>(d_src, d_dst, d_w, vertices, edges, d_dist, passes);
+    });
+}
+
+int sssp_bellman_ford_f64(const int32_t *d_src, const int32_t *d_dst, const double *d_w,
+                          int32_t vertices, int64_t edges, int32_t source, double *d_dist,
+                          int32_t *passes)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_dist && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst && d_w)),
+                     "bad arguments");
+        SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        ref_bf<WF64, unsigned long long>(d_src, d_dst,
+                                         reinterpret_cast<const unsigned long long *>(d_w), vertices,
+                                         edges, reinterpret_cast<unsigned long long *>(d_dist),
+                                         passes);
+    });
+}
+
+int sssp_collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *features,
+                         float *times_ms, int32_t *rows)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && rows && cap >= 0 && (cap == 0 || (features && times_ms)), "bad arguments");
+        SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
+        set_device(g->device);
+        g->have_result = false;
+        collect_timings(g, source, cap, features, times_ms, rows);
+        g->last_source = source;
+        g->have_result = true;
+    });
+}
+
+int sssp_features(double frontier_vertices, double frontier_edges, double unsettled_frac,
+                  double round_index, double mean_degree, double vertices, double edges,
+                  float *out5)
+{
+    return guard([&] {
+        SSSP_REQUIRE(out5, "null output");
+        make_features(frontier_vertices, frontier_edges, unsettled_frac, round_index, mean_degree,
+                      vertices, edges, out5);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1091 @@
+// engine.hip -- the relaxation kernels and the round loop of the MI355X
+// Bellman-Ford SSSP.
+//
+// One solve = distance init + relaxation rounds until the frontier is empty
+// and (by default) one full all-edge round confirms the fixpoint. Each round
+// runs in one of three modes, chosen per round on the device:
+//   PUSH  sparse frontier push: the frontier queue is compacted into entries
+//         (vertex, row start, exclusive degree scan) and edges are dealt out
+//         in fixed tiles of 256*K edges (merge-path style load balancing, so a
+//         hub of degree 10^5 is spread over many workgroups);
+//   ALL   all-edge sweep over the CSR of every vertex with out-degree > 0,
+//         relaxing the out-edges of active vertices (stamped in the previous
+//         or the current round, so updates propagate within the sweep);
+//   PULL  for every unsettled vertex (d > min over the frontier, which no
+//         future update can undercut) the min over its in-edges (CSC), with a
+//         segmented wave reduction before one atomicMin per vertex.
+// Relaxations use check-then-atomicMin on the 32-bit distance pattern. The
+// frontier of the next round is built with wave-aggregated appends and an
+// iteration stamp per vertex that deduplicates.
+//
+// Reference semantics followed (algorithm/sssp/gpu/bellman_ford.cu):
+//   init:      d[i] = max, d[source] = 0                      (:10-20, :38-56)
+//   relax:     if d[dst] > d[src] + w then d[dst] = d[src] + w  (:94-112)
+//   converge:  stop after a pass that changes nothing          (:126-141)
+// The plain schedule (SSSP_SCHED_PLAIN) is exactly that loop over the
+// src-sorted edge list, with the convergence flag tested on the device.
+#include "mlp_dev.hpp"
+#include "sssp_graph.hpp"
+
+namespace sssp {
+
+namespace {
+
+enum { DIR_PUSH = 0, DIR_PULL = 1, DIR_BFS = 2 };
+enum { SRC_SPARSE = 0, SRC_DENSE = 1 };
+enum { PRED_NONE = 0, PRED_STAMP = 1, PRED_REACHED = 2, PRED_UNSETTLED = 3 };
+
+constexpr int kLbItems = 8;
+constexpr int kLbTile = kBlock * kLbItems;  // queue entries per prep block
+constexpr int kSampleMax = 1 << 16;
+
+struct AdvArgs {
+    const int32_t *ent_v;      // entry vertex
+    const int64_t *ent_start;  // entry row start (sparse entries)
+    const int64_t *ent_scan;   // exclusive edge scan over entries, n+1 values
+    const int32_t *tile;       // tile -> first entry, ntiles+1 values
+    int32_t n;
+    int64_t T;                 // total edges over entries
+    int64_t ntiles;
+    const int64_t *row_ptr;    // out-degree lookup at append time
+    const int32_t *adj;        // el_dst (push) or csc_src (pull)
+    const uint32_t *wts;       // el_w or csc_w
+    uint32_t *dist;
+    int32_t *stamp;
+    int32_t *parent;           // certificate BFS
+    int32_t tick_prev;
+    int32_t tick_cur;
+    int32_t *qnext;
+    RoundSlot *slot;
+    uint32_t minF;
+};
+
+template <int DIR>
+__device__ __forceinline__ void append_frontier(bool low, int32_t v, const AdvArgs &a,
+                                                int64_t &qedges)
+{
+    bool app = false;
+    int64_t deg = 0;
+    if (low) {
+        deg = a.row_ptr[v + 1] - a.row_ptr[v];
+        if (deg > 0) {
+            if constexpr (DIR == DIR_BFS)
+                app = true;
+            else
+                app = atomicExch(&a.stamp[v], a.tick_cur) != a.tick_cur;
+        }
+    }
+    wave_append(app, v, a.qnext, &a.slot->qlen);
+    if (app) qedges += deg;
+}
+
+template <class WT, int DIR, int SRC, int PRED, int K>
+__global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
+{
+    constexpr int TE = kBlock * K;
+    constexpr uint32_t INF = WT::INF;
+    __shared__ int32_t s_off[TE + 1];
+    __shared__ int32_t s_v[TE + 1];
+    __shared__ uint32_t s_d[TE + 1];
+    __shared__ int64_t s_start[SRC == SRC_SPARSE ? TE + 1 : 1];
+
+    const int tid = threadIdx.x, lane = __lane_id();
+    int64_t relaxed = 0, qedges = 0;
+    int lowered_n = 0, reached_n = 0;
+    uint32_t minw = 0xFFFFFFFFu;
+
+    // XCD-aware tile order: groups of 16 consecutive tiles stay on one XCD
+    // (blocks are dispatched round-robin over the 8 XCDs); groups interleave
+    // across XCDs so skewed active regions still spread over the chip.
+    const int64_t x = blockIdx.x & 7;
+    const int64_t nbx = gridDim.x >> 3;
+    constexpr int64_t G = 16;
+    for (int64_t idx = blockIdx.x >> 3;; idx += nbx) {
+        const int64_t t = ((idx / G) * 8 + x) * G + (idx % G);
+        if (t >= a.ntiles) break;
+        const int64_t e0 = t * TE;
+        const int64_t e1 = min(e0 + (int64_t)TE, a.T);
+        const int32_t j0 = a.tile[t];
+        const int32_t jl = min(a.tile[t + 1], a.n - 1);
+        const int m = jl - j0 + 1;
+        int any = 0;
+        for (int jj = tid; jj < m; jj += kBlock) {
+            const int32_t j = j0 + jj;
+            s_off[jj] = (int32_t)(a.ent_scan[j] - e0);
+            const int32_t v = a.ent_v[j];
+            s_v[jj] = v;
+            if constexpr (SRC == SRC_SPARSE) s_start[jj] = a.ent_start[j];
+            const uint32_t dv = a.dist[v];
+            bool act;
+            if constexpr (DIR == DIR_PULL) {
+                act = dv > a.minF;
+                s_d[jj] = act ? dv : 0u;  // 0 marks "skip": no candidate is below 0
+            } else {
+                act = dv != INF;
+                if constexpr (PRED == PRED_STAMP) act = act && (a.stamp[v] >= a.tick_prev);
+                s_d[jj] = act ? dv : INF;
+            }
+            any |= act ? 1 : 0;
+        }
+        if (!__syncthreads_or(any)) continue;
+        int jj = 0;
+#pragma unroll 1
+        for (int k = 0; k < K; k++) {
+            const int32_t rel = k * kBlock + tid;
+            const bool valid = e0 + rel < e1;
+            int lo = jj, hi = m - 1;  // largest jj' with s_off[jj'] <= rel
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= rel) lo = mid;
+                else hi = mid - 1;
+            }
+            jj = lo;
+            if constexpr (DIR == DIR_PULL) {
+                const uint32_t dv = valid ? s_d[jj] : 0u;
+                uint32_t cand = INF;
+                if (dv != 0u) {
+                    const int64_t edge = (SRC == SRC_DENSE) ? e0 + rel : s_start[jj] + (rel - s_off[jj]);
+                    const int32_t u = a.adj[edge];
+                    const uint32_t du = a.dist[u];
+                    cand = (du == INF) ? INF : WT::add(du, a.wts[edge]);
+                    relaxed++;
+                }
+                // segmented min over lanes sharing an entry (jj is non-decreasing in lane)
+                const int seg = valid ? jj : (0x40000000 | lane);
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t oc = __shfl_down(cand, o);
+                    const int os = __shfl_down(seg, o);
+                    if (lane + o < 64 && os == seg) cand = min(cand, oc);
+                }
+                const int pseg = __shfl_up(seg, 1);
+                const bool head = (lane == 0) || (pseg != seg);
+                bool low = false;
+                int32_t v = 0;
+                if (head && dv != 0u && cand < dv) {
+                    v = s_v[jj];
+                    const uint32_t old = atomicMin(&a.dist[v], cand);
+                    if (cand < old) {
+                        low = true;
+                        minw = min(minw, cand);
+                        lowered_n++;
+                        reached_n += (old == INF);
+                    }
+                }
+                append_frontier<DIR>(low, v, a, qedges);
+            } else {
+                const uint32_t du = valid ? s_d[jj] : INF;
+                bool low = false;
+                int32_t v = 0;
+                if (du != INF) {
+                    const int64_t edge = (SRC == SRC_DENSE) ? e0 + rel : s_start[jj] + (rel - s_off[jj]);
+                    v = a.adj[edge];
+                    const uint32_t cand = WT::add(du, a.wts[edge]);
+                    relaxed++;
+                    const uint32_t dv = a.dist[v];
+                    if constexpr (DIR == DIR_BFS) {
+                        // certificate: tight edge into a finite vertex, first claimer wins
+                        if (dv != INF && cand == dv && atomicCAS(&a.parent[v], -1, s_v[jj]) == -1)
+                            low = true;
+                    } else {
+                        if (cand < dv) {
+                            const uint32_t old = atomicMin(&a.dist[v], cand);
+                            if (cand < old) {
+                                low = true;
+                                minw = min(minw, cand);
+                                lowered_n++;
+                                reached_n += (old == INF);
+                            }
+                        }
+                    }
+                }
+                append_frontier<DIR>(low, v, a, qedges);
+            }
+        }
+        __syncthreads();
+    }
+    relaxed = wave_sum64(relaxed);
+    qedges = wave_sum64(qedges);
+    lowered_n = wave_sum32(lowered_n);
+    reached_n = wave_sum32(reached_n);
+    minw = wave_min_u32(minw);
+    if (lane == 0) {
+        RoundSlot *s = a.slot;
+        if (relaxed) atomicAdd((unsigned long long *)&s->relaxed, (unsigned long long)relaxed);
+        if (qedges) atomicAdd((unsigned long long *)&s->qedges, (unsigned long long)qedges);
+        if (lowered_n) atomicAdd(&s->lowered, lowered_n);
+        if (reached_n) atomicAdd(&s->reached, reached_n);
+        if (minw != 0xFFFFFFFFu) atomicMin(&s->minw, minw);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sparse-frontier preparation: degree sums per block, block scan, entries.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_lb_reduce(const int32_t *q, int32_t n,
+                                                      const int64_t *row_ptr, int64_t *blk)
+{
+    __shared__ int64_t s_warp[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kLbTile + (int64_t)threadIdx.x * kLbItems;
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kLbItems; i++) {
+        const int64_t j = base + i;
+        if (j < n) {
+            const int32_t v = q[j];
+            s += row_ptr[v + 1] - row_ptr[v];
+        }
+    }
+    int64_t tot;
+    (void)block_excl_scan_i64(s, &tot, s_warp);
+    if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lb_write(const int32_t *q, int32_t n,
+                                                     const int64_t *row_ptr, const int64_t *blk,
+                                                     int32_t *lv, int64_t *ls, int64_t *lscan,
+                                                     int32_t *ltile, int64_t TE, int64_t ntiles)
+{
+    __shared__ int64_t s_warp[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kLbTile + (int64_t)threadIdx.x * kLbItems;
+    int32_t vv[kLbItems];
+    int64_t st[kLbItems], dg[kLbItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kLbItems; i++) {
+        const int64_t j = base + i;
+        vv[i] = 0;
+        st[i] = 0;
+        dg[i] = 0;
+        if (j < n) {
+            vv[i] = q[j];
+            st[i] = row_ptr[vv[i]];
+            dg[i] = row_ptr[vv[i] + 1] - st[i];
+        }
+        s += dg[i];
+    }
+    int64_t tot;
+    int64_t off = block_excl_scan_i64(s, &tot, s_warp) + blk[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kLbItems; i++) {
+        const int64_t j = base + i;
+        if (j < n) {
+            lv[j] = vv[i];
+            ls[j] = st[i];
+            lscan[j] = off;
+            for (int64_t t = (off + TE - 1) / TE; t * TE < off + dg[i]; t++) ltile[t] = (int32_t)j;
+            off += dg[i];
+            if (j == n - 1) {
+                lscan[n] = off;
+                ltile[ntiles] = n;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Init, round bookkeeping and the per-round decision.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_fill(T *p, int64_t n, T val)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        p[i] = val;
+}
+
+__device__ void zero_slot(RoundSlot *s, uint32_t minw)
+{
+    s->qlen = 0;
+    s->minw = minw;
+    s->qedges = 0;
+    s->relaxed = 0;
+    s->lowered = 0;
+    s->reached = 0;
+    s->sample_unsettled = 0;
+    s->sample_n = 0;
+    s->sample_unsettled_in = 0;
+}
+
+__global__ void k_init_solve(Ctrl *c, uint32_t *dist, int32_t *stamp, int32_t *q0,
+                             const int64_t *row_ptr, int32_t source, int32_t tick0)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    dist[source] = 0u;
+    const int64_t deg = row_ptr[source + 1] - row_ptr[source];
+    zero_slot(&c->slot[0], 0u);
+    zero_slot(&c->slot[1], 0xFFFFFFFFu);
+    c->slot[0].qlen = deg > 0 ? 1 : 0;
+    c->slot[0].qedges = deg;
+    if (deg > 0) {
+        q0[0] = source;
+        stamp[source] = tick0;
+    }
+    for (int i = 0; i < 64; i++) c->changed[i] = 0;
+    for (int i = 0; i < 8; i++) c->cert[i] = 0;
+    c->err = 0;
+}
+
+// Zeroes the slot the coming round fills and samples the unsettled fraction
+// of the frontier it consumes (d > minF, minF = min distance over the frontier).
+__global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r,
+                                                        const int32_t *sample_v, int32_t n_sample,
+                                                        const uint32_t *dist,
+                                                        const int64_t *col_ptr)
+{
+    RoundSlot *cur = &c->slot[r & 1];
+    RoundSlot *nxt = &c->slot[(r + 1) & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        zero_slot(nxt, 0xFFFFFFFFu);
+        cur->sample_n = n_sample;
+    }
+    const uint32_t minF = cur->minw;
+    int cnt = 0;
+    int64_t ind = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_sample;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int32_t v = sample_v[i];
+        if (dist[v] > minF) {
+            cnt++;
+            if (col_ptr) ind += col_ptr[v + 1] - col_ptr[v];
+        }
+    }
+    cnt = wave_sum32(cnt);
+    ind = wave_sum64(ind);
+    if (__lane_id() == 0) {
+        if (cnt) atomicAdd(&cur->sample_unsettled, cnt);
+        if (ind) atomicAdd((unsigned long long *)&cur->sample_unsettled_in, (unsigned long long)ind);
+    }
+}
+
+struct DecideArgs {
+    int32_t r;
+    int32_t sched;
+    double V, E;
+    int has_csc;
+    float push_edge_frac;
+    float pull_frac;
+    int push_chunk, all_chunk, pull_chunk;
+    const float *mlp;  // nullable
+};
+
+// One wave: features, the hand-written rule and (learned schedule) the MLP.
+__global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
+{
+    __shared__ float s_f[kIn];
+    __shared__ float s_h1[kHid];
+    __shared__ float s_h2[kHid];
+    __shared__ float s_o[kOut];
+    const int j = threadIdx.x;
+    const RoundSlot *cur = &c->slot[d.r & 1];
+    const double fv = cur->qlen, fe = (double)cur->qedges;
+    const double sn = cur->sample_n > 0 ? cur->sample_n : 1;
+    const double uns = cur->sample_unsettled / sn;
+    const double uns_in = (double)cur->sample_unsettled_in * (d.V / sn);
+    if (j == 0) {
+        float f[kIn];
+        make_features(fv, fe, uns, d.r, d.V > 0 ? d.E / d.V : 0.0, d.V, d.E, f);
+        for (int i = 0; i < kIn; i++) s_f[i] = f[i];
+    }
+    __syncthreads();
+    // hand-written rule: sparse push while the frontier's out-edges are a
+    // small fraction of E; pull when the unsettled in-edges are fewer than
+    // the frontier's out-edges; otherwise the all-edge sweep
+    int hmode, hchunk;
+    if (fe <= d.push_edge_frac * d.E) {
+        hmode = SSSP_MODE_PUSH;
+        hchunk = d.push_chunk;
+    } else if (d.has_csc && uns_in < d.pull_frac * fe) {
+        hmode = SSSP_MODE_PULL;
+        hchunk = d.pull_chunk;
+    } else {
+        hmode = SSSP_MODE_ALL;
+        hchunk = d.all_chunk;
+    }
+    int mode = hmode, chunk = hchunk, by_mlp = 0;
+    if (d.mlp && d.sched == SSSP_SCHED_LEARNED) {
+        const float *P = d.mlp;
+        float acc = P[kOffB1 + j];
+        for (int i = 0; i < kIn; i++) acc += P[kOffW1 + j * kIn + i] * s_f[i];
+        s_h1[j] = acc > 0.f ? acc : 0.f;
+        __syncthreads();
+        acc = P[kOffB2 + j];
+        for (int i = 0; i < kHid; i++) acc += P[kOffW2 + j * kHid + i] * s_h1[i];
+        s_h2[j] = acc > 0.f ? acc : 0.f;
+        __syncthreads();
+        if (j < kOut) {
+            acc = P[kOffB3 + j];
+            for (int i = 0; i < kHid; i++) acc += P[kOffW3 + j * kHid + i] * s_h2[i];
+            s_o[j] = acc;
+        }
+        __syncthreads();
+        bool ok = true;
+        for (int i = 0; i < kOut; i++) ok = ok && isfinite(s_o[i]);
+        if (ok) {
+            int bm = 0;
+            for (int i = 1; i < kModes; i++) {
+                if (i == SSSP_MODE_PULL && !d.has_csc) continue;
+                if (s_o[i] > s_o[bm]) bm = i;
+            }
+            int bc = 0;
+            for (int i = 1; i < SSSP_MLP_CHUNKS; i++)
+                if (s_o[kModes + i] > s_o[kModes + bc]) bc = i;
+            mode = bm;
+            chunk = bc;
+            by_mlp = 1;
+        }
+    }
+    if (j == 0) {
+        c->dec_mode = mode;
+        c->dec_chunk = chunk;
+        c->dec_mlp = by_mlp;
+        c->dec_heur_mode = hmode;
+        for (int i = 0; i < kIn; i++) c->feat[i] = s_f[i];
+        for (int i = 0; i < kOut; i++) c->logits[i] = by_mlp ? s_o[i] : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain Bellman-Ford pass over an edge list (bellman_ford.cu:62-99), with
+// check-then-atomicMin and the change flag kept on the device: pass p zeroes
+// the flag pass p+1 will set, and returns at once when pass p-1 changed
+// nothing, so several passes can be queued without a host round trip.
+// ---------------------------------------------------------------------------
+template <class WT, class DT, int VEC>
+__global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ src,
+                                                    const int32_t *__restrict__ dst,
+                                                    const DT *__restrict__ w, int64_t E,
+                                                    DT *dist, int32_t *changed, int32_t pass,
+                                                    int64_t per_block)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) changed[(pass + 1) & 63] = 0;
+    if (pass > 0 && __hip_atomic_load(&changed[(pass - 1) & 63], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == 0)
+        return;
+    const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+    bool low = false;
+    auto relax = [&](int32_t s, int32_t t, DT ww) {
+        const DT du = dist[s];
+        if (du == WT::INF) return;
+        const DT cand = WT::add(du, ww);
+        if (cand < dist[t]) {
+            const DT old = atomicMin(&dist[t], cand);
+            low |= cand < old;
+        }
+    };
+    if constexpr (VEC == 4) {
+        const int64_t ng = E >> 2;
+        const int64_t g0 = b * per_block, g1 = min(g0 + per_block, ng);
+        typedef int32_t v4i __attribute__((ext_vector_type(4)));
+        typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+        const v4i *s4 = reinterpret_cast<const v4i *>(src);
+        const v4i *t4 = reinterpret_cast<const v4i *>(dst);
+        const v4u *w4 = reinterpret_cast<const v4u *>(w);
+        for (int64_t g = g0 + threadIdx.x; g < g1; g += kBlock) {
+            const v4i s = __builtin_nontemporal_load(&s4[g]);
+            const v4i t = __builtin_nontemporal_load(&t4[g]);
+            const v4u ww = __builtin_nontemporal_load(&w4[g]);
+            relax(s.x, t.x, (DT)ww.x);
+            relax(s.y, t.y, (DT)ww.y);
+            relax(s.z, t.z, (DT)ww.z);
+            relax(s.w, t.w, (DT)ww.w);
+        }
+        if (b == 0 && threadIdx.x < (E & 3)) {
+            const int64_t e = (ng << 2) + threadIdx.x;
+            relax(src[e], dst[e], w[e]);
+        }
+    } else {
+        const int64_t e0 = b * per_block, e1 = min(e0 + per_block, E);
+        for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) relax(src[e], dst[e], w[e]);
+    }
+    if (__ballot(low) != 0 && __lane_id() == 0) changed[pass & 63] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Certificate helpers.
+// ---------------------------------------------------------------------------
+template <class WT>
+__global__ __launch_bounds__(kBlock) void k_cert_edges(const int32_t *src, const int32_t *dst,
+                                                       const uint32_t *w, int64_t E,
+                                                       const uint32_t *dist, int64_t *viol)
+{
+    int64_t n = 0;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * kBlock) {
+        const uint32_t du = dist[src[e]];
+        if (du == WT::INF) continue;
+        if (WT::add(du, w[e]) < dist[dst[e]]) n++;
+    }
+    n = wave_sum64(n);
+    if (__lane_id() == 0 && n) atomicAdd((unsigned long long *)viol, (unsigned long long)n);
+}
+
+__global__ void k_cert_root(Ctrl *c, int32_t *parent, int32_t *q0, const int64_t *row_ptr,
+                            const uint32_t *dist, int32_t source)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    parent[source] = source;  // visited marker during the BFS
+    const int64_t deg = row_ptr[source + 1] - row_ptr[source];
+    zero_slot(&c->slot[0], 0u);
+    zero_slot(&c->slot[1], 0xFFFFFFFFu);
+    c->slot[0].qlen = deg > 0 ? 1 : 0;
+    c->slot[0].qedges = deg;
+    if (deg > 0) q0[0] = source;
+    c->cert[1] = dist[source] != 0u ? 1 : 0;
+}
+
+template <class WT>
+__global__ __launch_bounds__(kBlock) void k_cert_count(const uint32_t *dist, int32_t *parent,
+                                                       int32_t V, int32_t source, int64_t *cert)
+{
+    int64_t reach = 0, untight = 0, tree = 0;
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * kBlock) {
+        const bool fin = dist[v] != WT::INF;
+        const int32_t p = parent[v];
+        reach += fin;
+        if (fin && p == -1) untight++;
+        if (p != -1) tree++;
+        if (v == source) parent[v] = -1;
+    }
+    reach = wave_sum64(reach);
+    untight = wave_sum64(untight);
+    tree = wave_sum64(tree);
+    if (__lane_id() == 0) {
+        if (untight) atomicAdd((unsigned long long *)&cert[2], (unsigned long long)untight);
+        if (reach) atomicAdd((unsigned long long *)&cert[5], (unsigned long long)reach);
+        if (tree) atomicAdd((unsigned long long *)&cert[7], (unsigned long long)tree);
+    }
+}
+
+inline unsigned grid_for(int64_t n, int64_t cap = 8192)
+{
+    int64_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    return (unsigned)(b < cap ? b : cap);
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+template <class WT, int DIR, int SRC, int PRED>
+void launch_adv_k(int kidx, unsigned grid, hipStream_t s, const AdvArgs &a)
+{
+    switch (kidx) {
+    case 0: k_advance<WT, DIR, SRC, PRED, 1><<<grid, kBlock, 0, s>>>(a); break;
+    case 1: k_advance<WT, DIR, SRC, PRED, 2><<<grid, kBlock, 0, s>>>(a); break;
+    case 2: k_advance<WT, DIR, SRC, PRED, 4><<<grid, kBlock, 0, s>>>(a); break;
+    default: k_advance<WT, DIR, SRC, PRED, 8><<<grid, kBlock, 0, s>>>(a); break;
+    }
+    SSSP_LAUNCH_CHECK("k_advance");
+}
+
+enum class Kind { SparsePush, DenseAll, DenseVerify, DensePull, SparseBfs };
+
+template <class WT>
+void launch_advance(sssp_graph *g, Kind kind, int kidx, AdvArgs &a)
+{
+    const int64_t TE = (int64_t)kBlock * kChunkK[kidx];
+    a.ntiles = (a.T + TE - 1) / TE;
+    if (a.ntiles <= 0 || a.n <= 0) return;
+    int64_t cap = g->advance_blocks > 0 ? g->advance_blocks : (int64_t)g->cu_count * 8;
+    cap = (cap + 7) / 8 * 8;
+    int64_t nb = (a.ntiles + 7) / 8 * 8;
+    if (nb > cap) nb = cap;
+    const unsigned grid = (unsigned)nb;
+    hipStream_t s = g->stream;
+    switch (kind) {
+    case Kind::SparsePush: launch_adv_k<WT, DIR_PUSH, SRC_SPARSE, PRED_NONE>(kidx, grid, s, a); break;
+    case Kind::DenseAll: launch_adv_k<WT, DIR_PUSH, SRC_DENSE, PRED_STAMP>(kidx, grid, s, a); break;
+    case Kind::DenseVerify: launch_adv_k<WT, DIR_PUSH, SRC_DENSE, PRED_REACHED>(kidx, grid, s, a); break;
+    case Kind::DensePull: launch_adv_k<WT, DIR_PULL, SRC_DENSE, PRED_UNSETTLED>(kidx, grid, s, a); break;
+    case Kind::SparseBfs: launch_adv_k<WT, DIR_BFS, SRC_SPARSE, PRED_NONE>(kidx, grid, s, a); break;
+    }
+}
+
+// Builds the entries of a sparse frontier (queue with out-degree > 0 only).
+void lb_prepare(sssp_graph *g, const int32_t *q, int32_t n, int64_t total, int kidx)
+{
+    hipStream_t s = g->stream;
+    const int64_t nb = (n + kLbTile - 1) / kLbTile;
+    if (nb + 1 > g->n_blk) throw Error{SSSP_ERR_INTERNAL, "lb scratch too small"};
+    const int64_t TE = (int64_t)kBlock * kChunkK[kidx];
+    const int64_t ntiles = (total + TE - 1) / TE;
+    k_lb_reduce<<<(unsigned)nb, kBlock, 0, s>>>(q, n, g->row_ptr, g->blk_a);
+    SSSP_LAUNCH_CHECK("k_lb_reduce");
+    scan_blocks_inplace(g->blk_a, nb, s);
+    k_lb_write<<<(unsigned)nb, kBlock, 0, s>>>(q, n, g->row_ptr, g->blk_a, g->lb_v, g->lb_start,
+                                                g->lb_scan, g->lb_tile, TE, ntiles);
+    SSSP_LAUNCH_CHECK("k_lb_write");
+}
+
+template <class WT>
+AdvArgs base_args(sssp_graph *g, int32_t r, int32_t tick0)
+{
+    AdvArgs a{};
+    a.row_ptr = g->row_ptr;
+    a.dist = g->dist;
+    a.stamp = g->stamp;
+    a.parent = g->parent;
+    a.tick_prev = tick0 + r;  // stamp of the frontier this round consumes
+    a.tick_cur = tick0 + r + 1;
+    a.qnext = g->queue[(r + 1) & 1];
+    a.slot = &g->ctrl->slot[(r + 1) & 1];
+    return a;
+}
+
+// Runs one relaxation round in the given mode.
+template <class WT>
+void run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, const RoundSlot &cur,
+               bool verify)
+{
+    AdvArgs a = base_args<WT>(g, r, tick0);
+    if (verify) {
+        a.ent_v = g->nzo_v;
+        a.ent_scan = g->nzo_scan;
+        a.tile = g->nzo_tile[kidx];
+        a.n = g->n_nzo;
+        a.T = g->E;
+        a.adj = g->el_dst;
+        a.wts = g->el_w;
+        launch_advance<WT>(g, Kind::DenseVerify, kidx, a);
+        return;
+    }
+    switch (mode) {
+    case SSSP_MODE_PUSH: {
+        const int32_t *q = g->queue[r & 1];
+        lb_prepare(g, q, cur.qlen, cur.qedges, kidx);
+        a.ent_v = g->lb_v;
+        a.ent_start = g->lb_start;
+        a.ent_scan = g->lb_scan;
+        a.tile = g->lb_tile;
+        a.n = cur.qlen;
+        a.T = cur.qedges;
+        a.adj = g->el_dst;
+        a.wts = g->el_w;
+        launch_advance<WT>(g, Kind::SparsePush, kidx, a);
+        break;
+    }
+    case SSSP_MODE_ALL: {
+        a.ent_v = g->nzo_v;
+        a.ent_scan = g->nzo_scan;
+        a.tile = g->nzo_tile[kidx];
+        a.n = g->n_nzo;
+        a.T = g->E;
+        a.adj = g->el_dst;
+        a.wts = g->el_w;
+        launch_advance<WT>(g, Kind::DenseAll, kidx, a);
+        break;
+    }
+    case SSSP_MODE_PULL: {
+        a.ent_v = g->nzi_v;
+        a.ent_scan = g->nzi_scan;
+        a.tile = g->nzi_tile[kidx];
+        a.n = g->n_nzi;
+        a.T = g->E;
+        a.adj = g->csc_src;
+        a.wts = g->csc_w;
+        a.minF = cur.minw;
+        launch_advance<WT>(g, Kind::DensePull, kidx, a);
+        break;
+    }
+    default: throw Error{SSSP_ERR_INTERNAL, "bad round mode"};
+    }
+}
+
+void read_slot(sssp_graph *g, int32_t r)
+{
+    hipStream_t s = g->stream;
+    Ctrl *d = g->ctrl;
+    Ctrl *h = g->ctrl_host;
+    SSSP_HIP(hipMemcpyAsync(&h->slot[r & 1], &d->slot[r & 1], sizeof(RoundSlot),
+                            hipMemcpyDeviceToHost, s));
+    SSSP_HIP(hipMemcpyAsync(&h->dec_mode, &d->dec_mode,
+                            sizeof(Ctrl) - offsetof(Ctrl, dec_mode), hipMemcpyDeviceToHost, s));
+    SSSP_HIP(hipStreamSynchronize(s));
+}
+
+template <class WT>
+void solve_plain(sssp_graph *g, sssp_stats *st)
+{
+    hipStream_t s = g->stream;
+    const int64_t E = g->E;
+    const int64_t ng = E >> 2;
+    int64_t nblocks = (int64_t)g->cu_count * 8;
+    if (nblocks < 8) nblocks = 8;
+    const int64_t per_block = (ng + nblocks - 1) / nblocks;
+    int32_t pass = 0;
+    const int batch = 4;
+    int32_t done_at = -1;
+    while (done_at < 0) {
+        for (int i = 0; i < batch; i++) {
+            k_el_pass<WT, uint32_t, 4><<<(unsigned)nblocks, kBlock, 0, s>>>(
+                g->el_src, g->el_dst, g->el_w, E, g->dist, g->ctrl->changed, pass + i, per_block);
+            SSSP_LAUNCH_CHECK("k_el_pass");
+        }
+        SSSP_HIP(hipMemcpyAsync(g->ctrl_host->changed, g->ctrl->changed, sizeof(int32_t) * 64,
+                                hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < batch; i++) {
+            if (g->ctrl_host->changed[(pass + i) & 63] == 0) {
+                done_at = pass + i;
+                break;
+            }
+        }
+        pass += batch;
+        if (done_at < 0 && pass > g->max_rounds)
+            throw Error{SSSP_ERR_NOT_CONVERGED, "plain Bellman-Ford did not converge within max_rounds"};
+        if (done_at < 0 && pass >= g->V + 1 && pass > 4)
+            throw Error{SSSP_ERR_NOT_CONVERGED, "plain Bellman-Ford exceeded V passes"};
+    }
+    st->iterations = done_at + 1;
+    st->n_all = done_at + 1;
+    st->verify_passes = 1;  // the last pass changed nothing: the fixpoint check
+    st->edges_relaxed = (int64_t)(done_at + 1) * E;
+}
+
+template <class WT>
+void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_stats *st)
+{
+    hipStream_t s = g->stream;
+    const bool learned = (sched == SSSP_SCHED_LEARNED);
+    const int ns = g->n_sample;
+    bool verified = false;
+    int64_t relaxed = 0;
+    int32_t reached = 1;
+    for (int32_t r = 0;; r++) {
+        if (r >= g->max_rounds)
+            throw Error{SSSP_ERR_NOT_CONVERGED, "solve did not converge within max_rounds"};
+        k_round_begin<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->sample_v, ns, g->dist,
+                                                          g->has_csc ? g->col_ptr : nullptr);
+        SSSP_LAUNCH_CHECK("k_round_begin");
+        DecideArgs da{};
+        da.r = r;
+        da.sched = sched;
+        da.V = g->V;
+        da.E = (double)g->E;
+        da.has_csc = g->has_csc ? 1 : 0;
+        da.push_edge_frac = (float)g->push_edge_frac;
+        da.pull_frac = (float)g->pull_frac;
+        da.push_chunk = g->push_chunk;
+        da.all_chunk = g->all_chunk;
+        da.pull_chunk = g->pull_chunk;
+        da.mlp = (learned && g->mlp) ? mlp_device_params(g->mlp) : nullptr;
+        k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
+        SSSP_LAUNCH_CHECK("k_decide");
+        read_slot(g, r);
+        const RoundSlot cur = g->ctrl_host->slot[r & 1];
+        relaxed += cur.relaxed;
+        reached += cur.reached;
+        bool verify = false;
+        if (cur.qlen == 0) {
+            if (!g->verify || verified) break;
+            verify = true;
+        }
+        int mode, kidx;
+        switch (sched) {
+        case SSSP_SCHED_PUSH: mode = SSSP_MODE_PUSH; kidx = g->push_chunk; break;
+        case SSSP_SCHED_ALL: mode = SSSP_MODE_ALL; kidx = g->all_chunk; break;
+        case SSSP_SCHED_PULL:
+            mode = g->has_csc ? SSSP_MODE_PULL : SSSP_MODE_ALL;
+            kidx = g->pull_chunk;
+            break;
+        default:
+            mode = g->ctrl_host->dec_mode;
+            kidx = g->ctrl_host->dec_chunk;
+            if (g->ctrl_host->dec_mlp) st->mlp_rounds++;
+            break;
+        }
+        if (mode == SSSP_MODE_PULL && !g->has_csc) mode = SSSP_MODE_ALL;
+        if (kidx < 0 || kidx >= kChunkCount) kidx = 1;
+        if (verify) {
+            mode = SSSP_MODE_ALL;
+            kidx = g->all_chunk;
+            st->verify_passes++;
+            verified = true;
+        } else {
+            verified = false;
+        }
+        if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
+        run_round<WT>(g, mode, kidx, r, tick0, cur, verify);
+        float ms = 0.f;
+        if (g->trace_events) {
+            SSSP_HIP(hipEventRecord(g->ev_r1, s));
+            SSSP_HIP(hipEventSynchronize(g->ev_r1));
+            SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
+        }
+        TraceRow tr{};
+        tr.mode = verify ? 3 : mode;
+        tr.chunk = kidx;
+        tr.frontier = cur.qlen;
+        tr.frontier_edges = verify ? g->E : cur.qedges;
+        tr.ms = ms;
+        tr.unsettled = cur.sample_n > 0 ? (float)cur.sample_unsettled / cur.sample_n : 0.f;
+        g->trace.push_back(tr);
+        st->iterations++;
+        if (mode == SSSP_MODE_PUSH) st->n_push++;
+        else if (mode == SSSP_MODE_ALL) st->n_all++;
+        else st->n_pull++;
+    }
+    st->edges_relaxed = relaxed;
+    st->reached = reached;
+}
+
+template <class WT>
+void solve_t(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
+{
+    hipStream_t s = g->stream;
+    if (g->tick > (1 << 30) - g->max_rounds - 4) {
+        k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->stamp, g->V, 0);
+        SSSP_LAUNCH_CHECK("stamp reset");
+        g->tick = 1;
+    }
+    const int32_t tick0 = g->tick;
+    g->trace.clear();
+    SSSP_HIP(hipEventRecord(g->ev_a, s));
+    k_fill<uint32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->V, WT::INF);
+    SSSP_LAUNCH_CHECK("k_fill dist");
+    k_init_solve<<<1, 64, 0, s>>>(g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, source, tick0);
+    SSSP_LAUNCH_CHECK("k_init_solve");
+    if (sched == SSSP_SCHED_PLAIN) {
+        solve_plain<WT>(g, st);
+        g->tick = tick0 + 1;
+    } else {
+        solve_rounds<WT>(g, source, sched, tick0, st);
+        g->tick = tick0 + st->iterations + 2;
+    }
+    SSSP_HIP(hipEventRecord(g->ev_b, s));
+    SSSP_HIP(hipEventSynchronize(g->ev_b));
+    float ms = 0.f;
+    SSSP_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
+    st->t_solve_ms = ms;
+    st->gteps = ms > 0 ? (double)g->E / (ms * 1e-3) / 1e9 : 0.0;
+    if (sched == SSSP_SCHED_PLAIN) {
+        // reached count for the plain schedule
+        st->reached = -1;
+    }
+}
+
+template <class WT>
+void certify_t(sssp_graph *g, int32_t source, int64_t *counts)
+{
+    hipStream_t s = g->stream;
+    Ctrl *c = g->ctrl;
+    k_fill<int64_t><<<1, 64, 0, s>>>(c->cert, 8, 0);
+    k_cert_edges<WT><<<grid_for(g->E), kBlock, 0, s>>>(g->el_src, g->el_dst, g->el_w, g->E, g->dist,
+                                                       &c->cert[0]);
+    SSSP_LAUNCH_CHECK("k_cert_edges");
+    k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->parent, g->V, -1);
+    k_cert_root<<<1, 64, 0, s>>>(c, g->parent, g->queue[0], g->row_ptr, g->dist, source);
+    SSSP_LAUNCH_CHECK("k_cert_root");
+    for (int32_t r = 0;; r++) {
+        if (r > g->V + 2) throw Error{SSSP_ERR_INTERNAL, "certificate BFS did not terminate"};
+        k_round_begin<<<1, kBlock, 0, s>>>(c, r, g->sample_v, 0, g->dist, nullptr);
+        SSSP_LAUNCH_CHECK("k_round_begin (bfs)");
+        read_slot(g, r);
+        const RoundSlot cur = g->ctrl_host->slot[r & 1];
+        if (cur.qlen == 0) break;
+        lb_prepare(g, g->queue[r & 1], cur.qlen, cur.qedges, 2);
+        AdvArgs a = base_args<WT>(g, r, 0);
+        a.ent_v = g->lb_v;
+        a.ent_start = g->lb_start;
+        a.ent_scan = g->lb_scan;
+        a.tile = g->lb_tile;
+        a.n = cur.qlen;
+        a.T = cur.qedges;
+        a.adj = g->el_dst;
+        a.wts = g->el_w;
+        launch_advance<WT>(g, Kind::SparseBfs, 2, a);
+    }
+    k_cert_count<WT><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->parent, g->V, source, c->cert);
+    SSSP_LAUNCH_CHECK("k_cert_count");
+    SSSP_HIP(hipMemcpyAsync(counts, c->cert, sizeof(int64_t) * 8, hipMemcpyDeviceToHost, s));
+    SSSP_HIP(hipStreamSynchronize(s));
+    // the BFS tree counts the source's own marker once: tree size = vertices
+    // with a parent + the source
+}
+
+__global__ void k_zero_slot(Ctrl *c, int32_t which)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) zero_slot(&c->slot[which & 1], 0xFFFFFFFFu);
+}
+
+// Timing collection: before every round, each mode x chunk bucket is run from
+// the same saved state (distances and stamps restored, next slot cleared) and
+// timed with device events; then the fastest is run for real.
+template <class WT>
+void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
+               int32_t *rows)
+{
+    hipStream_t s = g->stream;
+    if (g->tick > (1 << 30) - g->max_rounds - 4) {
+        k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->stamp, g->V, 0);
+        g->tick = 1;
+    }
+    const int32_t tick0 = g->tick;
+    const size_t V = (size_t)g->V;
+    uint32_t *sd = dmalloc<uint32_t>(V);
+    int32_t *sst = dmalloc<int32_t>(V);
+    struct Free {
+        void *a, *b;
+        ~Free()
+        {
+            dfree(a);
+            dfree(b);
+        }
+    } fr{sd, sst};
+    k_fill<uint32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->V, WT::INF);
+    k_init_solve<<<1, 64, 0, s>>>(g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, source, tick0);
+    SSSP_LAUNCH_CHECK("collect init");
+    const int nt = kModes * kChunkCount;
+    int32_t row = 0, r = 0;
+    for (;; r++) {
+        if (r >= g->max_rounds) throw Error{SSSP_ERR_NOT_CONVERGED, "collect: too many rounds"};
+        k_round_begin<<<grid_for(g->n_sample, 64), kBlock, 0, s>>>(
+            g->ctrl, r, g->sample_v, g->n_sample, g->dist, g->has_csc ? g->col_ptr : nullptr);
+        DecideArgs da{};
+        da.r = r;
+        da.sched = SSSP_SCHED_HEURISTIC;
+        da.V = g->V;
+        da.E = (double)g->E;
+        da.has_csc = g->has_csc;
+        da.push_edge_frac = (float)g->push_edge_frac;
+        da.pull_frac = (float)g->pull_frac;
+        k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
+        SSSP_LAUNCH_CHECK("collect decide");
+        read_slot(g, r);
+        const RoundSlot cur = g->ctrl_host->slot[r & 1];
+        if (cur.qlen == 0) break;
+        SSSP_HIP(hipMemcpyAsync(sd, g->dist, 4 * V, hipMemcpyDeviceToDevice, s));
+        SSSP_HIP(hipMemcpyAsync(sst, g->stamp, 4 * V, hipMemcpyDeviceToDevice, s));
+        float t[kModes * kChunkCount];
+        float best = 3.0e38f;
+        int bm = 0, bc = 0;
+        for (int mode = 0; mode < kModes; mode++) {
+            for (int c = 0; c < kChunkCount; c++) {
+                float ms = -1.f;
+                if (!(mode == SSSP_MODE_PULL && !g->has_csc)) {
+                    SSSP_HIP(hipMemcpyAsync(g->dist, sd, 4 * V, hipMemcpyDeviceToDevice, s));
+                    SSSP_HIP(hipMemcpyAsync(g->stamp, sst, 4 * V, hipMemcpyDeviceToDevice, s));
+                    k_zero_slot<<<1, 64, 0, s>>>(g->ctrl, r + 1);
+                    SSSP_HIP(hipEventRecord(g->ev_r0, s));
+                    run_round<WT>(g, mode, c, r, tick0, cur, false);
+                    SSSP_HIP(hipEventRecord(g->ev_r1, s));
+                    SSSP_HIP(hipEventSynchronize(g->ev_r1));
+                    SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
+                    if (ms < best) {
+                        best = ms;
+                        bm = mode;
+                        bc = c;
+                    }
+                }
+                t[mode * kChunkCount + c] = ms;
+            }
+        }
+        SSSP_HIP(hipMemcpyAsync(g->dist, sd, 4 * V, hipMemcpyDeviceToDevice, s));
+        SSSP_HIP(hipMemcpyAsync(g->stamp, sst, 4 * V, hipMemcpyDeviceToDevice, s));
+        k_zero_slot<<<1, 64, 0, s>>>(g->ctrl, r + 1);
+        run_round<WT>(g, bm, bc, r, tick0, cur, false);
+        if (row < cap) {
+            for (int i = 0; i < kIn; i++) feats[(size_t)row * kIn + i] = g->ctrl_host->feat[i];
+            for (int i = 0; i < nt; i++) times[(size_t)row * nt + i] = t[i];
+            row++;
+        }
+    }
+    SSSP_HIP(hipStreamSynchronize(s));
+    g->tick = tick0 + r + 2;
+    *rows = row;
+}
+
+}  // namespace
+
+void solve(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
+{
+    if (g->wtype == SSSP_W_F32) solve_t<WF32>(g, source, sched, st);
+    else solve_t<WU32>(g, source, sched, st);
+}
+
+void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
+                     int32_t *rows)
+{
+    if (g->wtype == SSSP_W_F32) collect_t<WF32>(g, source, cap, feats, times, rows);
+    else collect_t<WU32>(g, source, cap, feats, times, rows);
+}
+
+void certify(sssp_graph *g, int32_t source, int64_t *counts)
+{
+    if (g->wtype == SSSP_W_F32) certify_t<WF32>(g, source, counts);
+    else certify_t<WU32>(g, source, counts);
+}
+
+// ---------------------------------------------------------------------------
+// Reference-shaped wrappers over caller-owned device arrays.
+// ---------------------------------------------------------------------------
+template <class WT, class DT>
+void ref_init(DT *d, int32_t V, int32_t source)
+{
+    k_fill<DT><<<grid_for(V), kBlock>>>(d, V, WT::INF);
+    SSSP_LAUNCH_CHECK("init_distances");
+    k_fill<DT><<<1, 64>>>(d + source, 1, (DT)0);
+    SSSP_LAUNCH_CHECK("init_distances source");
+    SSSP_HIP(hipDeviceSynchronize());
+}
+
+template <class WT, class DT>
+void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int64_t E, DT *d,
+            int32_t *passes_out)
+{
+    int dev = 0;
+    SSSP_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SSSP_HIP(hipGetDeviceProperties(&prop, dev));
+    int32_t *changed = dmalloc<int32_t>(64);
+    int32_t hc[64];
+    struct Free {
+        void *p;
+        ~Free() { dfree(p); }
+    } fr{changed};
+    SSSP_HIP(hipMemset(changed, 0, sizeof(int32_t) * 64));
+    int64_t nblocks = (int64_t)prop.multiProcessorCount * 8;
+    const bool vec = sizeof(DT) == 4 && (reinterpret_cast<uintptr_t>(src) % 16 == 0) &&
+                     (reinterpret_cast<uintptr_t>(dst) % 16 == 0) &&
+                     (reinterpret_cast<uintptr_t>(w) % 16 == 0);
+    const int64_t units = vec ? (E >> 2) : E;
+    const int64_t per_block = (units + nblocks - 1) / nblocks;
+    int32_t pass = 0, done_at = -1;
+    while (done_at < 0) {
+        for (int i = 0; i < 4; i++) {
+            if (vec)
+                k_el_pass<WT, DT, 4><<<(unsigned)nblocks, kBlock>>>(src, dst, w, E, d, changed,
+                                                                    pass + i, per_block);
+            else
+                k_el_pass<WT, DT, 1><<<(unsigned)nblocks, kBlock>>>(src, dst, w, E, d, changed,
+                                                                    pass + i, per_block);
+            SSSP_LAUNCH_CHECK("bellman_ford pass");
+        }
+        SSSP_HIP(hipMemcpy(hc, changed, sizeof(hc), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 4; i++)
+            if (hc[(pass + i) & 63] == 0) {
+                done_at = pass + i;
+                break;
+            }
+        pass += 4;
+        // the reference loops at most V times (bellman_ford.cu:146)
+        if (done_at < 0 && pass >= V + 4)
+            throw Error{SSSP_ERR_NOT_CONVERGED, "Bellman-Ford did not converge within V passes"};
+    }
+    if (passes_out) *passes_out = done_at + 1;
+}
+
+template void ref_init<WF32, uint32_t>(uint32_t *, int32_t, int32_t);
+template void ref_init<WU32, uint32_t>(uint32_t *, int32_t, int32_t);
+template void ref_init<WF64, unsigned long long>(unsigned long long *, int32_t, int32_t);
+template void ref_bf<WF32, uint32_t>(const int32_t *, const int32_t *, const uint32_t *, int32_t,
+                                     int64_t, uint32_t *, int32_t *);
+template void ref_bf<WU32, uint32_t>(const int32_t *, const int32_t *, const uint32_t *, int32_t,
+                                     int64_t, uint32_t *, int32_t *);
+template void ref_bf<WF64, unsigned long long>(const int32_t *, const int32_t *,
+                                               const unsigned long long *, int32_t, int64_t,
+                                               unsigned long long *, int32_t *);
+
+}  // namespace sssp

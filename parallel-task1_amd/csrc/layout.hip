@@ -1,0 +1,433 @@
+// layout.hip -- builds the device layouts of a graph in HBM:
+//   * the edge list sorted by src, stable (el_src/el_dst/el_w) with row
+//     offsets row_ptr[V+1] (int64, as compressed_adjacency_list.h:13 keeps
+//     `long long *edge_ptrs`) -- this is the CSR of the reference
+//     (graph.hpp:316-361: per-source order = edge-list order);
+//   * the CSC (in-edges grouped by dst, stable) for pull rounds;
+//   * static lists of vertices with out-/in-degree > 0 and their tile tables.
+// Sorting is a stable LSD radix sort on the vertex id with the original edge
+// index as payload: per 4096-key tile an LDS digit histogram, one device-wide
+// scan of the (digit, tile) counts, then a scatter that ranks keys inside the
+// tile with 8 ballots per wave (stable by construction).
+#include "sssp_graph.hpp"
+
+#include <algorithm>
+
+namespace sssp {
+
+namespace {
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048 elements per block
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kBlock * kRsItems;  // 4096 keys per tile
+
+// --------------------------------------------------------------------------
+// Generic exclusive scan of int64, three phases.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int64_t *in, int64_t n, int64_t *blk)
+{
+    __shared__ int64_t s_warp[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++)
+        if (base + i < n) s += in[base + i];
+    int64_t tot;
+    (void)block_excl_scan_i64(s, &tot, s_warp);
+    if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of blk[0..nb) in place, blk[nb] = total
+__global__ __launch_bounds__(kBlock) void k_scan_blocks(int64_t *blk, int64_t nb)
+{
+    __shared__ int64_t s_warp[kBlock / 64];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += kBlock) {
+        int64_t i = base + threadIdx.x;
+        int64_t x = i < nb ? blk[i] : 0;
+        int64_t tot;
+        int64_t ex = block_excl_scan_i64(x, &tot, s_warp);
+        if (i < nb) blk[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) blk[nb] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(const int64_t *in, int64_t *out, int64_t n,
+                                                      const int64_t *blk)
+{
+    __shared__ int64_t s_warp[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int64_t v[kScanItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        v[i] = base + i < n ? in[base + i] : 0;
+        s += v[i];
+    }
+    int64_t tot;
+    int64_t run = block_excl_scan_i64(s, &tot, s_warp) + blk[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < n) out[base + i] = run;
+        run += v[i];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Input validation: ids in [0, V), weights >= 0 and not NaN.
+// --------------------------------------------------------------------------
+template <class W>
+__global__ __launch_bounds__(kBlock) void k_validate(const int32_t *src, const int32_t *dst,
+                                                     const W *w, int64_t E, int32_t V,
+                                                     int64_t *bad)
+{
+    int64_t nb_id = 0, nb_w = 0;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * kBlock) {
+        int32_t s = src[e], d = dst[e];
+        if (s < 0 || s >= V || d < 0 || d >= V) nb_id++;
+        W x = w[e];
+        if (!(x >= (W)0)) nb_w++;  // negative or NaN
+    }
+    nb_id = wave_sum64(nb_id);
+    nb_w = wave_sum64(nb_w);
+    if (__lane_id() == 0) {
+        if (nb_id) atomicAdd((unsigned long long *)&bad[0], (unsigned long long)nb_id);
+        if (nb_w) atomicAdd((unsigned long long *)&bad[1], (unsigned long long)nb_w);
+    }
+}
+
+// --------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value) by 8-bit digits.
+// counts layout: counts[digit * ntiles + tile] (digit-major) so that one
+// exclusive scan yields every tile's output base per digit, in tile order.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t *keys, int64_t n, int shift,
+                                                    int64_t *counts, int64_t ntiles)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+#pragma unroll 4
+    for (int i = 0; i < kRsItems; i++) {
+        int64_t idx = base + i * kBlock + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *keys_in,
+                                                       const uint32_t *vals_in,
+                                                       uint32_t *keys_out, uint32_t *vals_out,
+                                                       int64_t n, int shift,
+                                                       const int64_t *offsets, int64_t ntiles)
+{
+    __shared__ int64_t s_off[256];
+    __shared__ uint32_t s_cnt[kBlock / 64][256];
+    const int tid = threadIdx.x, lane = __lane_id(), wv = tid >> 6;
+    s_off[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) s_cnt[w][tid] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; r++) {
+        const int64_t idx = base + (int64_t)r * kBlock + tid;
+        const bool valid = idx < n;
+        uint32_t key = valid ? keys_in[idx] : 0u;
+        uint32_t val = valid ? (vals_in ? vals_in[idx] : (uint32_t)idx) : 0u;
+        uint32_t dg = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (dg >> b) & 1u;
+            uint64_t bm = __ballot(bit);
+            peers &= bit ? bm : ~bm;
+        }
+        const uint32_t rank = __popcll(peers & lanemask_lt());
+        if (valid && (peers >> lane) == 1ull) s_cnt[wv][dg] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wv; w++) pre += s_cnt[w][dg];
+            int64_t pos = s_off[dg] + pre + rank;
+            keys_out[pos] = key;
+            vals_out[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; w++) {
+            add += s_cnt[w][tid];
+            s_cnt[w][tid] = 0;
+        }
+        s_off[tid] += add;
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------
+// Layout assembly kernels.
+// --------------------------------------------------------------------------
+template <class W>
+__global__ __launch_bounds__(kBlock) void k_gather_pair(const uint32_t *perm, int64_t E,
+                                                        const int32_t *ids, const W *w,
+                                                        int32_t *ids_out, uint32_t *w_out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < E;
+         i += (int64_t)gridDim.x * kBlock) {
+        uint32_t p = perm[i];
+        ids_out[i] = ids[p];
+        W x = w[p];
+        uint32_t b;
+        if constexpr (sizeof(W) == 4) {
+            b = *reinterpret_cast<const uint32_t *>(&x);
+        } else {
+            b = 0;
+        }
+        w_out[i] = b;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_i32(const uint32_t *in, int32_t *out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        out[i] = (int32_t)in[i];
+}
+
+// row_ptr[v] = first position whose sorted key is >= v; row_ptr[V] = E.
+__global__ __launch_bounds__(kBlock) void k_rowptr_from_sorted(const uint32_t *keys, int64_t E,
+                                                               int32_t V, int64_t *row_ptr)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i <= E;
+         i += (int64_t)gridDim.x * kBlock) {
+        int64_t lo = (i == 0) ? 0 : (int64_t)keys[i - 1] + 1;  // first vertex not yet placed
+        int64_t hi = (i == E) ? (int64_t)V : (int64_t)keys[i];  // last vertex starting at i
+        for (int64_t v = lo; v <= hi; v++) row_ptr[v] = i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_nz_flags(const int64_t *ptr, int32_t V, int64_t *flags)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * kBlock)
+        flags[v] = ptr[v + 1] > ptr[v] ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nz_compact(const int64_t *ptr, int32_t V,
+                                                       const int64_t *pos, int32_t *nz_v,
+                                                       int64_t *nz_scan, int64_t n_nz, int64_t E)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * kBlock) {
+        if (ptr[v + 1] > ptr[v]) {
+            int64_t j = pos[v];
+            nz_v[j] = (int32_t)v;
+            nz_scan[j] = ptr[v];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) nz_scan[n_nz] = E;
+}
+
+// tile_out[t] = the entry whose edge range contains edge t * TE; tile_out[ntiles] = n.
+__global__ __launch_bounds__(kBlock) void k_tile_table(const int64_t *scan, int32_t n, int64_t TE,
+                                                       int64_t ntiles, int32_t *tile_out)
+{
+    for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n;
+         j += (int64_t)gridDim.x * kBlock) {
+        int64_t a = scan[j], b = scan[j + 1];
+        for (int64_t t = (a + TE - 1) / TE; t * TE < b; t++) tile_out[t] = (int32_t)j;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) tile_out[ntiles] = n;
+}
+
+inline int grid_for(int64_t n, int64_t cap = 8192)
+{
+    int64_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    return (int)(b < cap ? b : cap);
+}
+
+// Sorts by key; returns the permutation (original indices) in *perm_out
+// (one of the two value buffers) and sorted keys in *keys_out.
+void radix_sort_pairs(const int32_t *keys_in, int64_t n, int32_t max_key, uint32_t *kA,
+                      uint32_t *kB, uint32_t *vA, uint32_t *vB, int64_t *counts, int64_t *blk_a,
+                      int64_t *blk_b, int64_t n_blk, hipStream_t s, uint32_t **keys_out,
+                      uint32_t **perm_out)
+{
+    int bits = 0;
+    while (bits < 31 && ((int64_t)1 << bits) <= (int64_t)max_key) bits++;
+    int passes = (bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    const int64_t ntiles = (n + kRsTile - 1) / kRsTile;
+    const uint32_t *kin = reinterpret_cast<const uint32_t *>(keys_in);
+    const uint32_t *vin = nullptr;  // iota on the first pass
+    uint32_t *kout = kA, *vout = vA;
+    for (int p = 0; p < passes; p++) {
+        const int shift = 8 * p;
+        if (ntiles > 0) {
+            k_rs_hist<<<(unsigned)ntiles, kBlock, 0, s>>>(kin, n, shift, counts, ntiles);
+            SSSP_LAUNCH_CHECK("k_rs_hist");
+            device_exclusive_scan_i64(counts, counts, 256 * ntiles, blk_a, blk_b, n_blk, s);
+            k_rs_scatter<<<(unsigned)ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, n, shift,
+                                                              counts, ntiles);
+            SSSP_LAUNCH_CHECK("k_rs_scatter");
+        }
+        kin = kout;
+        vin = vout;
+        kout = (kout == kA) ? kB : kA;
+        vout = (vout == vA) ? vB : vA;
+    }
+    *keys_out = const_cast<uint32_t *>(kin);
+    *perm_out = const_cast<uint32_t *>(vin);
+}
+
+void build_static_list(const int64_t *ptr, int32_t V, int64_t E, int64_t *tmp_flags,
+                       int64_t *tmp_pos, int64_t *blk_a, int64_t *blk_b, int64_t n_blk,
+                       hipStream_t s, int32_t **nz_v, int64_t **nz_scan, int32_t *n_nz,
+                       int32_t **tiles)
+{
+    k_nz_flags<<<grid_for(V), kBlock, 0, s>>>(ptr, V, tmp_flags);
+    SSSP_LAUNCH_CHECK("k_nz_flags");
+    device_exclusive_scan_i64(tmp_flags, tmp_pos, V, blk_a, blk_b, n_blk, s);
+    int64_t last_pos = 0, last_flag = 0;
+    if (V > 0) {
+        SSSP_HIP(hipMemcpyAsync(&last_pos, tmp_pos + V - 1, 8, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipMemcpyAsync(&last_flag, tmp_flags + V - 1, 8, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+    }
+    const int64_t n = last_pos + last_flag;
+    *n_nz = (int32_t)n;
+    *nz_v = dmalloc<int32_t>((size_t)n + 1);
+    *nz_scan = dmalloc<int64_t>((size_t)n + 1);
+    k_nz_compact<<<grid_for(V), kBlock, 0, s>>>(ptr, V, tmp_pos, *nz_v, *nz_scan, n, E);
+    SSSP_LAUNCH_CHECK("k_nz_compact");
+    for (int c = 0; c < kChunkCount; c++) {
+        const int64_t TE = (int64_t)kBlock * kChunkK[c];
+        const int64_t ntiles = (E + TE - 1) / TE;
+        tiles[c] = dmalloc<int32_t>((size_t)ntiles + 1);
+        build_tile_table(*nz_scan, (int32_t)n, E, (int32_t)TE, tiles[c], s);
+    }
+}
+
+}  // namespace
+
+void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s)
+{
+    k_scan_blocks<<<1, kBlock, 0, s>>>(blk, nb);
+    SSSP_LAUNCH_CHECK("k_scan_blocks");
+}
+
+int64_t scan_blocks_needed(int64_t n) { return (n + kScanTile - 1) / kScanTile + 2; }
+
+void device_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *blk_a,
+                               int64_t *blk_b, int64_t n_blk_cap, hipStream_t s)
+{
+    (void)blk_b;
+    if (n <= 0) return;
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb + 1 > n_blk_cap) throw Error{SSSP_ERR_INTERNAL, "scan scratch too small"};
+    k_scan_reduce<<<(unsigned)nb, kBlock, 0, s>>>(in, n, blk_a);
+    SSSP_LAUNCH_CHECK("k_scan_reduce");
+    k_scan_blocks<<<1, kBlock, 0, s>>>(blk_a, nb);
+    SSSP_LAUNCH_CHECK("k_scan_blocks");
+    k_scan_down<<<(unsigned)nb, kBlock, 0, s>>>(in, out, n, blk_a);
+    SSSP_LAUNCH_CHECK("k_scan_down");
+}
+
+void build_tile_table(const int64_t *scan, int32_t n, int64_t total, int32_t tile_edges,
+                      int32_t *tile_out, hipStream_t s)
+{
+    const int64_t ntiles = (total + tile_edges - 1) / tile_edges;
+    k_tile_table<<<grid_for(n > 0 ? n : 1), kBlock, 0, s>>>(scan, n, tile_edges, ntiles, tile_out);
+    SSSP_LAUNCH_CHECK("k_tile_table");
+}
+
+void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
+                   bool build_csc)
+{
+    hipStream_t s = g->stream;
+    const int64_t E = g->E;
+    const int32_t V = g->V;
+
+    // validation
+    {
+        int64_t *bad = dmalloc<int64_t>(2);
+        SSSP_HIP(hipMemsetAsync(bad, 0, 16, s));
+        if (E > 0) {
+            if (g->wtype == SSSP_W_F32)
+                k_validate<float><<<grid_for(E), kBlock, 0, s>>>(
+                    d_src, d_dst, reinterpret_cast<const float *>(d_w), E, V, bad);
+            else
+                k_validate<uint32_t><<<grid_for(E), kBlock, 0, s>>>(d_src, d_dst, d_w, E, V, bad);
+            SSSP_LAUNCH_CHECK("k_validate");
+        }
+        int64_t hb[2] = {0, 0};
+        SSSP_HIP(hipMemcpyAsync(hb, bad, 16, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        dfree(bad);
+        if (hb[0]) throw Error{SSSP_ERR_ARG, std::to_string(hb[0]) + " edges have a vertex id outside [0, V)"};
+        if (hb[1]) throw Error{SSSP_ERR_ARG, std::to_string(hb[1]) + " edges have a negative or NaN weight"};
+    }
+
+    // scratch for the sorts
+    const int64_t rs_tiles = (E + kRsTile - 1) / kRsTile;
+    const int64_t scan_n = std::max<int64_t>((int64_t)V + 1, 256 * rs_tiles);
+    g->n_blk = scan_blocks_needed(scan_n);
+    g->blk_a = dmalloc<int64_t>((size_t)g->n_blk);
+    g->blk_b = dmalloc<int64_t>((size_t)g->n_blk);
+    uint32_t *kA = dmalloc<uint32_t>((size_t)E), *kB = dmalloc<uint32_t>((size_t)E);
+    uint32_t *vA = dmalloc<uint32_t>((size_t)E), *vB = dmalloc<uint32_t>((size_t)E);
+    int64_t *counts = dmalloc<int64_t>((size_t)std::max<int64_t>(256 * rs_tiles, (int64_t)V + 1));
+    int64_t *tmp_pos = dmalloc<int64_t>((size_t)V + 1);
+    struct Scratch {
+        void *p[6];
+        ~Scratch()
+        {
+            for (void *x : p) dfree(x);
+        }
+    } scratch{{kA, kB, vA, vB, counts, tmp_pos}};
+
+    // ---- CSR / src-sorted edge list ----
+    uint32_t *skeys = nullptr, *perm = nullptr;
+    radix_sort_pairs(d_src, E, V > 0 ? V - 1 : 0, kA, kB, vA, vB, counts, g->blk_a, g->blk_b,
+                     g->n_blk, s, &skeys, &perm);
+    g->el_src = dmalloc<int32_t>((size_t)E);
+    g->el_dst = dmalloc<int32_t>((size_t)E);
+    g->el_w = dmalloc<uint32_t>((size_t)E);
+    g->row_ptr = dmalloc<int64_t>((size_t)V + 1);
+    if (E > 0) {
+        k_copy_i32<<<grid_for(E), kBlock, 0, s>>>(skeys, g->el_src, E);
+        k_gather_pair<uint32_t><<<grid_for(E), kBlock, 0, s>>>(perm, E, d_dst, d_w, g->el_dst, g->el_w);
+        SSSP_LAUNCH_CHECK("csr gather");
+    }
+    k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(skeys, E, V, g->row_ptr);
+    SSSP_LAUNCH_CHECK("k_rowptr_from_sorted");
+    build_static_list(g->row_ptr, V, E, counts, tmp_pos, g->blk_a, g->blk_b, g->n_blk, s,
+                      &g->nzo_v, &g->nzo_scan, &g->n_nzo, g->nzo_tile);
+
+    // ---- CSC ----
+    if (build_csc) {
+        radix_sort_pairs(d_dst, E, V > 0 ? V - 1 : 0, kA, kB, vA, vB, counts, g->blk_a, g->blk_b,
+                         g->n_blk, s, &skeys, &perm);
+        g->csc_src = dmalloc<int32_t>((size_t)E);
+        g->csc_w = dmalloc<uint32_t>((size_t)E);
+        g->col_ptr = dmalloc<int64_t>((size_t)V + 1);
+        if (E > 0) {
+            k_gather_pair<uint32_t><<<grid_for(E), kBlock, 0, s>>>(perm, E, d_src, d_w, g->csc_src, g->csc_w);
+            SSSP_LAUNCH_CHECK("csc gather");
+        }
+        k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(skeys, E, V, g->col_ptr);
+        SSSP_LAUNCH_CHECK("k_rowptr_from_sorted csc");
+        build_static_list(g->col_ptr, V, E, counts, tmp_pos, g->blk_a, g->blk_b, g->n_blk, s,
+                          &g->nzi_v, &g->nzi_scan, &g->n_nzi, g->nzi_tile);
+        g->has_csc = true;
+    }
+    SSSP_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace sssp

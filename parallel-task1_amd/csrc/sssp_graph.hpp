@@ -1,0 +1,187 @@
+// sssp_graph.hpp -- the device-resident graph object and the kernels' shared
+// launch helpers. Internal to the library.
+#pragma once
+
+#include "sssp_internal.hpp"
+
+namespace sssp {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 waves)
+constexpr int kMaxRounds = 1 << 16;  // hard cap on relaxation rounds per solve
+constexpr int kChunkCount = SSSP_MLP_CHUNKS;
+// chunk width bucket -> edges per lane per tile (tile = 256 * K edges)
+constexpr int kChunkK[kChunkCount] = {1, 2, 4, 8};
+
+// Per-round counters written by the relaxation kernels. Two slots indexed by
+// round parity: round r writes slot (r+1)&1 (the frontier it builds) while
+// the host reads slot r&1 (the frontier it consumes).
+struct alignas(64) RoundSlot {
+    int32_t qlen;          // vertices appended (out-degree > 0)
+    uint32_t minw;         // min distance written this round (bit pattern)
+    int64_t qedges;        // sum of out-degrees of appended vertices
+    int64_t relaxed;       // edge relaxations evaluated
+    int32_t lowered;       // successful atomicMin lowerings
+    int32_t reached;       // first reaches (INF -> finite)
+    // filled by the round-begin kernel for the consuming round:
+    int32_t sample_unsettled;   // sampled vertices with d > minF
+    int32_t sample_n;           // sample size
+    int64_t sample_unsettled_in;  // sum of in-degrees of the unsettled sample
+};
+
+struct Ctrl {
+    RoundSlot slot[2];
+    int32_t changed[64];   // plain passes: changed[p & 63] = 1 when pass p lowered a distance
+    int64_t n_compact;     // lb scan: entries
+    int64_t lb_total;      // lb scan: total edges
+    int64_t cert[8];       // certificate counters
+    int32_t err;           // device-side error flag
+    // per-round decision written by the decision kernel
+    int32_t dec_mode;
+    int32_t dec_chunk;
+    int32_t dec_mlp;       // 1 when the network made the choice
+    int32_t dec_heur_mode;
+    float feat[SSSP_MLP_IN];
+    float logits[SSSP_MLP_OUT];
+};
+
+struct TraceRow {
+    int32_t mode;
+    int32_t chunk;
+    int64_t frontier;
+    int64_t frontier_edges;
+    float ms;
+    float unsettled;
+};
+
+// mlp.hip
+const float *mlp_device_params(sssp_mlp *m);
+
+}  // namespace sssp
+
+struct sssp_graph {
+    int device = 0;
+    int wtype = SSSP_W_F32;
+    int32_t V = 0;
+    int64_t E = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+
+    // src-sorted edge list == CSR (row_ptr, el_dst, el_w) plus the explicit src
+    int64_t *row_ptr = nullptr;
+    int32_t *el_src = nullptr;
+    int32_t *el_dst = nullptr;
+    uint32_t *el_w = nullptr;
+    // CSC: in-edges grouped by dst
+    int64_t *col_ptr = nullptr;
+    int32_t *csc_src = nullptr;
+    uint32_t *csc_w = nullptr;
+    bool has_csc = false;
+    // static entry lists of vertices with out- (in-) degree > 0; *_scan has
+    // n+1 entries = their row starts followed by E
+    int32_t *nzo_v = nullptr;
+    int64_t *nzo_scan = nullptr;
+    int32_t n_nzo = 0;
+    int32_t *nzi_v = nullptr;
+    int64_t *nzi_scan = nullptr;
+    int32_t n_nzi = 0;
+    // tile -> first entry tables of the static lists, one per chunk bucket
+    int32_t *nzo_tile[sssp::kChunkCount] = {};
+    int32_t *nzi_tile[sssp::kChunkCount] = {};
+    // vertices sampled for the unsettled-fraction estimate
+    int32_t *sample_v = nullptr;
+    int32_t n_sample = 0;
+
+    // solve state
+    uint32_t *dist = nullptr;
+    int32_t *stamp = nullptr;
+    int32_t *queue[2] = {nullptr, nullptr};
+    int32_t *lb_v = nullptr;      // compacted frontier (sparse push)
+    int64_t *lb_start = nullptr;  // row start per entry
+    int64_t *lb_scan = nullptr;   // exclusive degree scan, n+1
+    int32_t *lb_tile = nullptr;   // tile -> first entry
+    int64_t *blk_a = nullptr;     // block partials for scans
+    int64_t *blk_b = nullptr;
+    int64_t n_blk = 0;
+    int32_t *parent = nullptr;    // certificate tight tree
+    sssp::Ctrl *ctrl = nullptr;
+    sssp::Ctrl *ctrl_host = nullptr;  // pinned mirror
+    int32_t tick = 1;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_r0 = nullptr, ev_r1 = nullptr;
+
+    // knobs
+    int push_chunk = 2, all_chunk = 2, pull_chunk = 2;
+    int advance_blocks = 0;  // 0 = derive from the CU count
+    int verify = 1;
+    int max_rounds = 100000;
+    double push_edge_frac = 0.05;  // heuristic: sparse push below this frontier-edge fraction
+    double pull_frac = 0.5;        // heuristic: pull when unsettled in-edges < pull_frac * frontier edges
+    int trace_events = 0;
+
+    // last solve
+    int32_t last_source = -1;
+    int last_sched = -1;
+    bool have_result = false;
+    std::vector<sssp::TraceRow> trace;
+
+    sssp_mlp *mlp = nullptr;
+
+    // batch state (allocated on demand)
+    uint32_t *bdist = nullptr;
+    int64_t bdist_elems = 0;
+    int32_t *bscratch = nullptr;
+    int64_t bscratch_elems = 0;
+
+    int64_t device_bytes = 0;
+};
+
+namespace sssp {
+
+// Block-wide exclusive scan of one int64 per thread (kBlock threads);
+// s_warp must hold kBlock/64 entries. Contains barriers: call uniformly.
+__device__ inline int64_t block_excl_scan_i64(int64_t x, int64_t *total, int64_t *s_warp)
+{
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    int64_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_warp[w] = inc;
+    __syncthreads();
+    int64_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) {
+        if (i < w) wpre += s_warp[i];
+        tot += s_warp[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return wpre + inc - x;
+}
+
+// layout.hip
+void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s);
+void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
+                   bool build_csc);
+void device_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *blk_a,
+                               int64_t *blk_b, int64_t n_blk_cap, hipStream_t s);
+int64_t scan_blocks_needed(int64_t n);
+void build_tile_table(const int64_t *scan, int32_t n, int64_t total, int32_t tile_edges,
+                      int32_t *tile_out, hipStream_t s);
+
+// engine.hip
+template <class WT, class DT>
+void ref_init(DT *d, int32_t V, int32_t source);
+template <class WT, class DT>
+void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int64_t E, DT *d,
+            int32_t *passes_out);
+void solve(sssp_graph *g, int32_t source, int sched, sssp_stats *st);
+void certify(sssp_graph *g, int32_t source, int64_t *counts);
+void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
+                     int32_t *rows);
+
+// mlp.hip
+const float *mlp_device_params(sssp_mlp *m);
+
+}  // namespace sssp

@@ -1,0 +1,137 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact distances for f32 and u32 (the fixpoint of d[v] <= d[u] (+) w
+is unique for w >= 0, so every schedule must land on the same bits as the
+oracle's restatement of cpu_bellman_ford, sssp.hpp:3-71), plus the
+shortest-path certificate (all edges satisfied, tight parent tree).
+"""
+import numpy as np
+import pytest
+
+import sssp_amd as S
+from conftest import bits, small_graphs
+
+SCHEDS = ["plain", "heuristic", "learned", "push", "all", "pull"]
+
+
+def _check_solution(g, oracle, src, dst, w, V, source, d, sched):
+    ref = oracle.dijkstra(src, dst, w, V, source)
+    assert np.array_equal(bits(d), bits(ref)), f"{sched}: distances differ from the oracle"
+    ok, counts, par = g.certify(source)
+    assert ok, f"{sched}: GPU certificate failed: {counts}"
+    rc, oc = oracle.certify(src, dst, w, V, source, d, par)
+    assert rc == 0, f"{sched}: oracle rejects the GPU parents/distances: {oc}"
+
+
+@pytest.mark.gpu
+def test_device_is_gfx950(gpu):
+    info = S.device_info(0)
+    print(info)
+    assert info["arch"].startswith("gfx950")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,V,src,dst,w", small_graphs(), ids=[g[0] for g in small_graphs()])
+def test_small_graphs_every_schedule(gpu, oracle, name, V, src, dst, w):
+    with S.DeviceGraph(src, dst, w, V) as g:
+        for sched in SCHEDS:
+            d, st = g.solve(0, sched)
+            _check_solution(g, oracle, src, dst, w, V, 0, d, sched)
+    if name == "chain":
+        assert np.array_equal(d, (np.arange(10) * np.float32(0.5)).astype(np.float32))
+    if name == "disconnected":
+        assert np.all(d[3:] == S.F32_INF)
+
+
+@pytest.mark.gpu
+def test_layout_is_stable_sort(gpu):
+    src, dst, w = S.rmat(11, 16, seed=3, threads=4)
+    V = 1 << 11
+    with S.DeviceGraph(src, dst, w, V) as g:
+        L = g.layout()
+    p = np.argsort(src, kind="stable")
+    assert np.array_equal(L["el_src"], src[p])
+    assert np.array_equal(L["el_dst"], dst[p])
+    assert np.array_equal(bits(L["el_w"]), bits(w[p]))
+    rp = np.concatenate([[0], np.cumsum(np.bincount(src, minlength=V))])
+    assert np.array_equal(L["row_ptr"], rp)
+    q = np.argsort(dst, kind="stable")
+    assert np.array_equal(L["csc_src"], src[q])
+    assert np.array_equal(bits(L["csc_w"]), bits(w[q]))
+    cp = np.concatenate([[0], np.cumsum(np.bincount(dst, minlength=V))])
+    assert np.array_equal(L["col_ptr"], cp)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scale,ef,seed,threads", [(10, 16, 1, 64), (14, 16, 9, 64), (16, 16, 0, 1)])
+def test_rmat_every_schedule_bit_exact(gpu, oracle, scale, ef, seed, threads):
+    src, dst, w = S.rmat(scale, ef, seed=seed, threads=threads)
+    V = 1 << scale
+    sources = [0] + list(S.pick_sources(src, V, seed + 100, 3))
+    with S.DeviceGraph(src, dst, w, V) as g:
+        for s in sources:
+            ref, _ = oracle.bf(src, dst, w, V, s)
+            for sched in SCHEDS:
+                d, st = g.solve(s, sched)
+                assert np.array_equal(bits(d), bits(ref)), (s, sched, st)
+            ok, counts, par = g.certify(s)
+            assert ok, counts
+            rc, oc = oracle.certify(src, dst, w, V, s, d, par)
+            assert rc == 0, oc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("logV,deg", [(12, 16), (15, 8)])
+def test_er_u32_bit_exact(gpu, oracle, logV, deg):
+    V = 1 << logV
+    src, dst, w = S.er(V, deg, seed=3, weight_mode=1)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        ref, _ = oracle.bf(src, dst, w, V, 0)
+        dj = oracle.dijkstra(src, dst, w, V, 0)
+        assert np.array_equal(ref, dj)
+        for sched in SCHEDS:
+            d, st = g.solve(0, sched)
+            assert d.dtype == np.uint32
+            assert np.array_equal(d, ref), sched
+        ok, counts, par = g.certify(0)
+        assert ok, counts
+        rc, oc = oracle.certify(src, dst, w, V, 0, d, par)
+        assert rc == 0, oc
+
+
+@pytest.mark.gpu
+def test_edge_order_does_not_change_distances(gpu):
+    src, dst, w = S.rmat(13, 16, seed=4, threads=8)
+    V = 1 << 13
+    perm = np.random.default_rng(0).permutation(len(src))
+    with S.DeviceGraph(src, dst, w, V) as g1, S.DeviceGraph(src[perm], dst[perm], w[perm], V) as g2:
+        for sched in ("plain", "heuristic"):
+            a, _ = g1.solve(5, sched)
+            b, _ = g2.solve(5, sched)
+            assert np.array_equal(bits(a), bits(b))
+
+
+@pytest.mark.gpu
+def test_repeat_solves_reuse_graph(gpu, oracle):
+    """Stamps and ticks carry across solves on one graph; results must not."""
+    src, dst, w = S.rmat(12, 8, seed=7, threads=8)
+    V = 1 << 12
+    with S.DeviceGraph(src, dst, w, V) as g:
+        for s in [0, 1, 2, 0, 3]:
+            for sched in ("push", "all", "pull", "heuristic"):
+                d, _ = g.solve(s, sched)
+                ref = oracle.dijkstra(src, dst, w, V, s)
+                assert np.array_equal(bits(d), bits(ref))
+
+
+@pytest.mark.gpu
+def test_verify_pass_and_trace(gpu):
+    src, dst, w = S.rmat(12, 16, seed=2, threads=8)
+    with S.DeviceGraph(src, dst, w, 1 << 12) as g:
+        g.set_param("trace_events", 1)
+        d, st = g.solve(0, "heuristic")
+        tr = g.trace()
+        assert st["verify_passes"] >= 1
+        assert tr["mode"][-1] == S.MODE_VERIFY
+        assert len(tr["mode"]) == st["iterations"]
+        assert st["t_solve_ms"] > 0
