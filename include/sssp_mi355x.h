@@ -104,6 +104,15 @@ int sssp_device_info(int32_t device, char *name, int32_t name_len, int32_t *cu_c
 int sssp_graph_create(int32_t weight_type, const int32_t *src, const int32_t *dst,
                       const void *weights, int32_t vertices, int64_t edges,
                       int32_t device, uint32_t flags, sssp_graph **out);
+/* Generates a directed R-MAT graph on the device -- bit-identical to
+ * sssp_rmat_generate with the same arguments (the LCG behind rand_r is jumped
+ * ahead per edge) -- and builds the graph from it without a host round trip.
+ * src_out/dst_out/w_out (host, 2^scale*edge_factor each, may be NULL) receive
+ * the generated edge list in generation order. */
+int sssp_graph_create_rmat(int32_t scale, int32_t edge_factor, int32_t a, int32_t b, int32_t c,
+                           int32_t d, uint32_t base_seed, int32_t virtual_threads, int32_t device,
+                           uint32_t flags, int32_t *src_out, int32_t *dst_out, float *w_out,
+                           sssp_graph **out);
 int sssp_graph_destroy(sssp_graph *g);
 int sssp_graph_info(const sssp_graph *g, int32_t *vertices, int64_t *edges,
                     int32_t *weight_type, int64_t *device_bytes);

@@ -71,6 +71,39 @@ int orc_bf_f32(const int32_t *src, const int32_t *dst, const float *w,
     return iterations;
 }
 
+/* The same loop stopped after at most `max_passes` passes (bounded CPU
+ * baseline samples); returns the passes run, *converged = 1 when the last
+ * pass changed nothing. */
+int orc_bf_f32_capped(const int32_t *src, const int32_t *dst, const float *w,
+                      int32_t V, int64_t E, int32_t source, int32_t threads,
+                      int32_t max_passes, float *dist, int32_t *converged)
+{
+    if (threads <= 0) threads = omp_get_max_threads();
+    volatile int changes = 1;
+    int iterations = 0;
+    #pragma omp parallel num_threads(threads) shared(changes, iterations)
+    {
+        #pragma omp for schedule(static)
+        for (int32_t i = 0; i < V; i++) dist[i] = FLT_MAX;
+        #pragma omp single
+        dist[source] = 0.0f;
+        while (changes && iterations < max_passes) {
+            #pragma omp barrier
+            #pragma omp single
+            { iterations++; changes = 0; }
+            #pragma omp for schedule(guided)
+            for (int64_t e = 0; e < E; e++) {
+                float ds = dist[src[e]];
+                float dd = dist[dst[e]];
+                float cand = ds + w[e];
+                if (dd > cand) { dist[dst[e]] = cand; changes = 1; }
+            }
+        }
+    }
+    *converged = changes ? 0 : 1;
+    return iterations;
+}
+
 int orc_bf_f64(const int32_t *src, const int32_t *dst, const double *w,
                int32_t V, int64_t E, int32_t source, int32_t threads,
                double *dist)

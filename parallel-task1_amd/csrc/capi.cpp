@@ -102,6 +102,82 @@ int sssp_device_info(int32_t device, char *name, int32_t name_len, int32_t *cu_c
     });
 }
 
+}  // extern "C"
+
+namespace sssp {
+namespace {
+
+sssp_graph *graph_begin(int32_t weight_type, int32_t vertices, int64_t edges, int32_t device)
+{
+    SSSP_REQUIRE(weight_type == SSSP_W_F32 || weight_type == SSSP_W_U32,
+                 "graph weight type must be SSSP_W_F32 or SSSP_W_U32");
+    SSSP_REQUIRE(vertices >= 1, "graph needs at least one vertex");
+    SSSP_REQUIRE(edges >= 0 && edges < ((int64_t)1 << 31) - 1, "edge count must be in [0, 2^31 - 1)");
+    set_device(device);
+    sssp_graph *g = new sssp_graph();
+    g->device = device;
+    g->wtype = weight_type;
+    g->V = vertices;
+    g->E = edges;
+    hipDeviceProp_t p;
+    SSSP_HIP(hipGetDeviceProperties(&p, device));
+    g->cu_count = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+    SSSP_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    return g;
+}
+
+// Builds the layouts from edge arrays already in device memory and allocates
+// the solve state.
+void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
+                  uint32_t flags)
+{
+    hipStream_t s = g->stream;
+    build_layouts(g, d_src, d_dst, d_w, !(flags & SSSP_NO_CSC));
+    const size_t V = (size_t)g->V;
+    g->dist = dmalloc<uint32_t>(V);
+    g->stamp = dmalloc<int32_t>(V);
+    SSSP_HIP(hipMemsetAsync(g->stamp, 0, 4 * V, s));
+    g->queue[0] = dmalloc<int32_t>(V + 1);
+    g->queue[1] = dmalloc<int32_t>(V + 1);
+    g->lb_v = dmalloc<int32_t>(V + 1);
+    g->lb_start = dmalloc<int64_t>(V + 1);
+    g->lb_scan = dmalloc<int64_t>(V + 2);
+    g->lb_tile = dmalloc<int32_t>((size_t)(g->E / kBlock) + 4);
+    g->parent = dmalloc<int32_t>(V);
+    g->ctrl = dmalloc<Ctrl>(1);
+    SSSP_HIP(hipMemsetAsync(g->ctrl, 0, sizeof(Ctrl), s));
+    SSSP_HIP(hipHostMalloc((void **)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
+    std::memset(g->ctrl_host, 0, sizeof(Ctrl));
+    // fixed-stride vertex sample for the unsettled-fraction estimate
+    g->n_sample = g->V < 16384 ? g->V : 16384;
+    std::vector<int32_t> sv((size_t)g->n_sample);
+    for (int32_t i = 0; i < g->n_sample; i++)
+        sv[(size_t)i] = (int32_t)(((int64_t)i * g->V) / g->n_sample);
+    g->sample_v = dmalloc<int32_t>((size_t)g->n_sample);
+    SSSP_HIP(hipMemcpyAsync(g->sample_v, sv.data(), 4 * sv.size(), hipMemcpyHostToDevice, s));
+    SSSP_HIP(hipEventCreate(&g->ev_a));
+    SSSP_HIP(hipEventCreate(&g->ev_b));
+    SSSP_HIP(hipEventCreate(&g->ev_r0));
+    SSSP_HIP(hipEventCreate(&g->ev_r1));
+    SSSP_HIP(hipStreamSynchronize(s));
+    const int64_t Ei = g->E, Vi = g->V;
+    g->device_bytes = Ei * 12 + (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) +
+                      Vi * 4 * 7 + Vi * 16 + Vi * 16;
+}
+
+struct DevBuf {
+    void *p[3] = {nullptr, nullptr, nullptr};
+    ~DevBuf()
+    {
+        for (void *x : p) dfree(x);
+    }
+};
+
+}  // namespace
+}  // namespace sssp
+
+extern "C" {
+
 int sssp_graph_create(int32_t weight_type, const int32_t *src, const int32_t *dst,
                       const void *weights, int32_t vertices, int64_t edges, int32_t device,
                       uint32_t flags, sssp_graph **out)
@@ -110,42 +186,18 @@ int sssp_graph_create(int32_t weight_type, const int32_t *src, const int32_t *ds
     int rc = guard([&] {
         SSSP_REQUIRE(out, "null output handle");
         *out = nullptr;
-        SSSP_REQUIRE(weight_type == SSSP_W_F32 || weight_type == SSSP_W_U32,
-                     "graph weight type must be SSSP_W_F32 or SSSP_W_U32");
-        SSSP_REQUIRE(vertices >= 1, "graph needs at least one vertex");
-        SSSP_REQUIRE(edges >= 0 && edges < ((int64_t)1 << 31) - 1,
-                     "edge count must be in [0, 2^31 - 1)");
         SSSP_REQUIRE(edges == 0 || (src && dst && weights), "null edge array");
-        set_device(device);
-        g = new sssp_graph();
-        g->device = device;
-        g->wtype = weight_type;
-        g->V = vertices;
-        g->E = edges;
-        hipDeviceProp_t p;
-        SSSP_HIP(hipGetDeviceProperties(&p, device));
-        g->cu_count = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
-        SSSP_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        g = graph_begin(weight_type, vertices, edges, device);
         hipStream_t s = g->stream;
-
-        // upload (or use the caller's device arrays)
         const int32_t *d_src = src, *d_dst = dst;
         const uint32_t *d_w = static_cast<const uint32_t *>(weights);
-        int32_t *u_src = nullptr, *u_dst = nullptr;
-        uint32_t *u_w = nullptr;
-        struct Up {
-            void *p[3];
-            ~Up()
-            {
-                for (void *x : p) dfree(x);
-            }
-        } up{{nullptr, nullptr, nullptr}};
+        DevBuf up;
         if (!(flags & SSSP_INPUT_DEVICE) && edges > 0) {
-            u_src = dmalloc<int32_t>((size_t)edges);
+            int32_t *u_src = dmalloc<int32_t>((size_t)edges);
             up.p[0] = u_src;
-            u_dst = dmalloc<int32_t>((size_t)edges);
+            int32_t *u_dst = dmalloc<int32_t>((size_t)edges);
             up.p[1] = u_dst;
-            u_w = dmalloc<uint32_t>((size_t)edges);
+            uint32_t *u_w = dmalloc<uint32_t>((size_t)edges);
             up.p[2] = u_w;
             SSSP_HIP(hipMemcpyAsync(u_src, src, 4 * (size_t)edges, hipMemcpyHostToDevice, s));
             SSSP_HIP(hipMemcpyAsync(u_dst, dst, 4 * (size_t)edges, hipMemcpyHostToDevice, s));
@@ -154,39 +206,42 @@ int sssp_graph_create(int32_t weight_type, const int32_t *src, const int32_t *ds
             d_dst = u_dst;
             d_w = u_w;
         }
-        build_layouts(g, d_src, d_dst, d_w, !(flags & SSSP_NO_CSC));
+        graph_finish(g, d_src, d_dst, d_w, flags);
+        *out = g;
+    });
+    if (rc != SSSP_OK && g) graph_free(g);
+    return rc;
+}
 
-        // solve state
-        const size_t V = (size_t)vertices;
-        g->dist = dmalloc<uint32_t>(V);
-        g->stamp = dmalloc<int32_t>(V);
-        SSSP_HIP(hipMemsetAsync(g->stamp, 0, 4 * V, s));
-        g->queue[0] = dmalloc<int32_t>(V + 1);
-        g->queue[1] = dmalloc<int32_t>(V + 1);
-        g->lb_v = dmalloc<int32_t>(V + 1);
-        g->lb_start = dmalloc<int64_t>(V + 1);
-        g->lb_scan = dmalloc<int64_t>(V + 2);
-        g->lb_tile = dmalloc<int32_t>((size_t)(edges / kBlock) + 4);
-        g->parent = dmalloc<int32_t>(V);
-        g->ctrl = dmalloc<Ctrl>(1);
-        SSSP_HIP(hipMemsetAsync(g->ctrl, 0, sizeof(Ctrl), s));
-        SSSP_HIP(hipHostMalloc((void **)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
-        std::memset(g->ctrl_host, 0, sizeof(Ctrl));
-        // fixed-stride vertex sample for the unsettled-fraction estimate
-        g->n_sample = vertices < 16384 ? vertices : 16384;
-        std::vector<int32_t> sv((size_t)g->n_sample);
-        for (int32_t i = 0; i < g->n_sample; i++)
-            sv[(size_t)i] = (int32_t)(((int64_t)i * vertices) / g->n_sample);
-        g->sample_v = dmalloc<int32_t>((size_t)g->n_sample);
-        SSSP_HIP(hipMemcpyAsync(g->sample_v, sv.data(), 4 * sv.size(), hipMemcpyHostToDevice, s));
-        SSSP_HIP(hipEventCreate(&g->ev_a));
-        SSSP_HIP(hipEventCreate(&g->ev_b));
-        SSSP_HIP(hipEventCreate(&g->ev_r0));
-        SSSP_HIP(hipEventCreate(&g->ev_r1));
-        SSSP_HIP(hipStreamSynchronize(s));
-        const int64_t Ei = edges, Vi = vertices;
-        g->device_bytes = Ei * 12 + (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) +
-                          Vi * 4 * 7 + Vi * 16 + Vi * 16;
+int sssp_graph_create_rmat(int32_t scale, int32_t edge_factor, int32_t a, int32_t b, int32_t c,
+                           int32_t d, uint32_t base_seed, int32_t virtual_threads, int32_t device,
+                           uint32_t flags, int32_t *src_out, int32_t *dst_out, float *w_out,
+                           sssp_graph **out)
+{
+    sssp_graph *g = nullptr;
+    int rc = guard([&] {
+        SSSP_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        SSSP_REQUIRE(scale >= 1 && scale <= 30, "rmat: scale must be in [1, 30]");
+        SSSP_REQUIRE(edge_factor >= 1, "rmat: edge_factor must be >= 1");
+        SSSP_REQUIRE(a >= 0 && b >= 0 && c >= 0 && d >= 0 && a + b + c + d <= 100,
+                     "rmat: probabilities are integer percents summing to <= 100");
+        SSSP_REQUIRE(virtual_threads >= 1, "rmat: virtual_threads must be >= 1");
+        const int64_t E = ((int64_t)1 << scale) * edge_factor;
+        g = graph_begin(SSSP_W_F32, 1 << scale, E, device);
+        hipStream_t s = g->stream;
+        DevBuf gen;
+        int32_t *d_src = dmalloc<int32_t>((size_t)E);
+        gen.p[0] = d_src;
+        int32_t *d_dst = dmalloc<int32_t>((size_t)E);
+        gen.p[1] = d_dst;
+        float *d_w = dmalloc<float>((size_t)E);
+        gen.p[2] = d_w;
+        rmat_device(scale, edge_factor, a, b, c, d, base_seed, virtual_threads, d_src, d_dst, d_w, s);
+        if (src_out) SSSP_HIP(hipMemcpyAsync(src_out, d_src, 4 * (size_t)E, hipMemcpyDeviceToHost, s));
+        if (dst_out) SSSP_HIP(hipMemcpyAsync(dst_out, d_dst, 4 * (size_t)E, hipMemcpyDeviceToHost, s));
+        if (w_out) SSSP_HIP(hipMemcpyAsync(w_out, d_w, 4 * (size_t)E, hipMemcpyDeviceToHost, s));
+        graph_finish(g, d_src, d_dst, reinterpret_cast<const uint32_t *>(d_w), flags);
         *out = g;
     });
     if (rc != SSSP_OK && g) graph_free(g);

@@ -60,23 +60,70 @@ struct AdvArgs {
     uint32_t minF;
 };
 
+constexpr int kStageCap = 512;  // per-wave LDS staging slots for frontier appends
+
+// Per-block accumulators of one round, reduced once per block at the end.
+struct RoundAcc {
+    int64_t relaxed = 0, qedges = 0;
+    int lowered = 0, reached = 0;
+    uint32_t minw = 0xFFFFFFFFu;
+};
+
+__device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
+{
+    __shared__ int64_t r_rel[kBlock / 64], r_qe[kBlock / 64];
+    __shared__ int r_low[kBlock / 64], r_rea[kBlock / 64];
+    __shared__ uint32_t r_min[kBlock / 64];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    const int64_t rel = wave_sum64(acc.relaxed), qe = wave_sum64(acc.qedges);
+    const int low = wave_sum32(acc.lowered), rea = wave_sum32(acc.reached);
+    const uint32_t mn = wave_min_u32(acc.minw);
+    if (lane == 0) {
+        r_rel[w] = rel;
+        r_qe[w] = qe;
+        r_low[w] = low;
+        r_rea[w] = rea;
+        r_min[w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t a = 0, b = 0;
+        int c = 0, d = 0;
+        uint32_t m = 0xFFFFFFFFu;
+        for (int i = 0; i < kBlock / 64; i++) {
+            a += r_rel[i];
+            b += r_qe[i];
+            c += r_low[i];
+            d += r_rea[i];
+            m = r_min[i] < m ? r_min[i] : m;
+        }
+        if (a) atomicAdd((unsigned long long *)&s->relaxed, (unsigned long long)a);
+        if (b) atomicAdd((unsigned long long *)&s->qedges, (unsigned long long)b);
+        if (c) atomicAdd(&s->lowered, c);
+        if (d) atomicAdd(&s->reached, d);
+        if (m != 0xFFFFFFFFu) atomicMin(&s->minw, m);
+    }
+}
+
+// A vertex whose distance this lane lowered joins the next frontier when it
+// has out-edges and no other lane appended it in this round (stamp).
 template <int DIR>
-__device__ __forceinline__ void append_frontier(bool low, int32_t v, const AdvArgs &a,
-                                                int64_t &qedges)
+__device__ __forceinline__ void append_frontier(bool low, int32_t v, const int64_t *row_ptr,
+                                                int32_t *stamp, int32_t tick_cur, int64_t &qedges,
+                                                int32_t *wbuf, int &wcnt)
 {
     bool app = false;
-    int64_t deg = 0;
     if (low) {
-        deg = a.row_ptr[v + 1] - a.row_ptr[v];
+        const int64_t deg = row_ptr[v + 1] - row_ptr[v];
         if (deg > 0) {
             if constexpr (DIR == DIR_BFS)
                 app = true;
             else
-                app = atomicExch(&a.stamp[v], a.tick_cur) != a.tick_cur;
+                app = atomicExch(&stamp[v], tick_cur) != tick_cur;
         }
+        if (app) qedges += deg;
     }
-    wave_append(app, v, a.qnext, &a.slot->qlen);
-    if (app) qedges += deg;
+    stage_push(app, v, wbuf, wcnt);
 }
 
 template <class WT, int DIR, int SRC, int PRED, int K>
@@ -88,11 +135,12 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
     __shared__ int32_t s_v[TE + 1];
     __shared__ uint32_t s_d[TE + 1];
     __shared__ int64_t s_start[SRC == SRC_SPARSE ? TE + 1 : 1];
+    __shared__ int32_t s_stage[kBlock / 64][kStageCap];
 
     const int tid = threadIdx.x, lane = __lane_id();
-    int64_t relaxed = 0, qedges = 0;
-    int lowered_n = 0, reached_n = 0;
-    uint32_t minw = 0xFFFFFFFFu;
+    int32_t *wbuf = s_stage[tid >> 6];
+    int wcnt = 0;
+    RoundAcc acc;
 
     // XCD-aware tile order: groups of 16 consecutive tiles stay on one XCD
     // (blocks are dispatched round-robin over the 8 XCDs); groups interleave
@@ -140,6 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                 else hi = mid - 1;
             }
             jj = lo;
+            bool low = false;
+            int32_t v = 0;
             if constexpr (DIR == DIR_PULL) {
                 const uint32_t dv = valid ? s_d[jj] : 0u;
                 uint32_t cand = INF;
@@ -148,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                     const int32_t u = a.adj[edge];
                     const uint32_t du = a.dist[u];
                     cand = (du == INF) ? INF : WT::add(du, a.wts[edge]);
-                    relaxed++;
+                    acc.relaxed++;
                 }
                 // segmented min over lanes sharing an entry (jj is non-decreasing in lane)
                 const int seg = valid ? jj : (0x40000000 | lane);
@@ -160,28 +210,23 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                 }
                 const int pseg = __shfl_up(seg, 1);
                 const bool head = (lane == 0) || (pseg != seg);
-                bool low = false;
-                int32_t v = 0;
                 if (head && dv != 0u && cand < dv) {
                     v = s_v[jj];
                     const uint32_t old = atomicMin(&a.dist[v], cand);
                     if (cand < old) {
                         low = true;
-                        minw = min(minw, cand);
-                        lowered_n++;
-                        reached_n += (old == INF);
+                        acc.minw = min(acc.minw, cand);
+                        acc.lowered++;
+                        acc.reached += (old == INF);
                     }
                 }
-                append_frontier<DIR>(low, v, a, qedges);
             } else {
                 const uint32_t du = valid ? s_d[jj] : INF;
-                bool low = false;
-                int32_t v = 0;
                 if (du != INF) {
                     const int64_t edge = (SRC == SRC_DENSE) ? e0 + rel : s_start[jj] + (rel - s_off[jj]);
                     v = a.adj[edge];
                     const uint32_t cand = WT::add(du, a.wts[edge]);
-                    relaxed++;
+                    acc.relaxed++;
                     const uint32_t dv = a.dist[v];
                     if constexpr (DIR == DIR_BFS) {
                         // certificate: tight edge into a finite vertex, first claimer wins
@@ -192,31 +237,117 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                             const uint32_t old = atomicMin(&a.dist[v], cand);
                             if (cand < old) {
                                 low = true;
-                                minw = min(minw, cand);
-                                lowered_n++;
-                                reached_n += (old == INF);
+                                acc.minw = min(acc.minw, cand);
+                                acc.lowered++;
+                                acc.reached += (old == INF);
                             }
                         }
                     }
                 }
-                append_frontier<DIR>(low, v, a, qedges);
             }
+            append_frontier<DIR>(low, v, a.row_ptr, a.stamp, a.tick_cur, acc.qedges, wbuf, wcnt);
+            if (wcnt > kStageCap - 64) stage_flush(wbuf, wcnt, a.qnext, &a.slot->qlen);
         }
         __syncthreads();
     }
-    relaxed = wave_sum64(relaxed);
-    qedges = wave_sum64(qedges);
-    lowered_n = wave_sum32(lowered_n);
-    reached_n = wave_sum32(reached_n);
-    minw = wave_min_u32(minw);
-    if (lane == 0) {
-        RoundSlot *s = a.slot;
-        if (relaxed) atomicAdd((unsigned long long *)&s->relaxed, (unsigned long long)relaxed);
-        if (qedges) atomicAdd((unsigned long long *)&s->qedges, (unsigned long long)qedges);
-        if (lowered_n) atomicAdd(&s->lowered, lowered_n);
-        if (reached_n) atomicAdd(&s->reached, reached_n);
-        if (minw != 0xFFFFFFFFu) atomicMin(&s->minw, minw);
+    stage_flush(wbuf, wcnt, a.qnext, &a.slot->qlen);
+    flush_acc(acc, a.slot);
+}
+
+// ---------------------------------------------------------------------------
+// All-edge round over the src-sorted edge list: the plain pass's streaming
+// access (16-byte loads, 4 edges per lane) restricted to active sources
+// (PRED_STAMP: stamped in the previous or the current round; PRED_REACHED:
+// any finite distance -- the fixpoint check), with frontier appends.
+// ---------------------------------------------------------------------------
+template <class WT, int PRED>
+__global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict__ src,
+                                                      const int32_t *__restrict__ dst,
+                                                      const uint32_t *__restrict__ w, int64_t E,
+                                                      uint32_t *dist, int32_t *stamp,
+                                                      int32_t tick_prev, int32_t tick_cur,
+                                                      const int64_t *row_ptr, int32_t *qnext,
+                                                      RoundSlot *slot, int64_t per_block)
+{
+    typedef int32_t v4i __attribute__((ext_vector_type(4)));
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    constexpr uint32_t INF = WT::INF;
+    __shared__ int32_t s_stage[kBlock / 64][kStageCap];
+    int32_t *wbuf = s_stage[threadIdx.x >> 6];
+    int wcnt = 0;
+    RoundAcc acc;
+    const int64_t ng = E >> 2;
+    const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t g0 = b * per_block, g1 = min(g0 + per_block, ng);
+    const int64_t iters = g1 > g0 ? (g1 - g0 + kBlock - 1) / kBlock : 0;
+    const v4i *s4 = reinterpret_cast<const v4i *>(src);
+    const v4i *t4 = reinterpret_cast<const v4i *>(dst);
+    const v4u *w4 = reinterpret_cast<const v4u *>(w);
+    for (int64_t it = 0; it <= iters; it++) {
+        // the last iteration handles the E % 4 tail edges in block 0
+        int32_t es[4], et[4];
+        uint32_t ew[4];
+        bool val[4] = {false, false, false, false};
+        if (it < iters) {
+            const int64_t g = g0 + it * kBlock + threadIdx.x;
+            if (g < g1) {
+                const v4i sv = __builtin_nontemporal_load(&s4[g]);
+                es[0] = sv.x; es[1] = sv.y; es[2] = sv.z; es[3] = sv.w;
+                val[0] = val[1] = val[2] = val[3] = true;
+            }
+        } else if (b == 0 && threadIdx.x < (E & 3)) {
+            es[0] = src[(ng << 2) + threadIdx.x];
+            val[0] = true;
+        }
+        uint32_t du[4];
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            du[i] = INF;
+            if (val[i]) {
+                const uint32_t d = dist[es[i]];
+                bool act = d != INF;
+                if constexpr (PRED == PRED_STAMP) act = act && stamp[es[i]] >= tick_prev;
+                du[i] = act ? d : INF;
+                any |= act;
+            }
+        }
+        if (any) {
+            if (it < iters) {
+                const int64_t g = g0 + it * kBlock + threadIdx.x;
+                const v4i tv = __builtin_nontemporal_load(&t4[g]);
+                const v4u wv = __builtin_nontemporal_load(&w4[g]);
+                et[0] = tv.x; et[1] = tv.y; et[2] = tv.z; et[3] = tv.w;
+                ew[0] = wv.x; ew[1] = wv.y; ew[2] = wv.z; ew[3] = wv.w;
+            } else {
+                et[0] = dst[(ng << 2) + threadIdx.x];
+                ew[0] = w[(ng << 2) + threadIdx.x];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            bool low = false;
+            int32_t v = 0;
+            if (du[i] != INF) {
+                v = et[i];
+                const uint32_t cand = WT::add(du[i], ew[i]);
+                acc.relaxed++;
+                if (cand < dist[v]) {
+                    const uint32_t old = atomicMin(&dist[v], cand);
+                    if (cand < old) {
+                        low = true;
+                        acc.minw = min(acc.minw, cand);
+                        acc.lowered++;
+                        acc.reached += (old == INF);
+                    }
+                }
+            }
+            append_frontier<DIR_PUSH>(low, v, row_ptr, stamp, tick_cur, acc.qedges, wbuf, wcnt);
+        }
+        if (wcnt > kStageCap - 256) stage_flush(wbuf, wcnt, qnext, &slot->qlen);
     }
+    stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+    flush_acc(acc, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -580,7 +711,7 @@ void launch_adv_k(int kidx, unsigned grid, hipStream_t s, const AdvArgs &a)
     SSSP_LAUNCH_CHECK("k_advance");
 }
 
-enum class Kind { SparsePush, DenseAll, DenseVerify, DensePull, SparseBfs };
+enum class Kind { SparsePush, DensePull, SparseBfs };
 
 template <class WT>
 void launch_advance(sssp_graph *g, Kind kind, int kidx, AdvArgs &a)
@@ -596,8 +727,6 @@ void launch_advance(sssp_graph *g, Kind kind, int kidx, AdvArgs &a)
     hipStream_t s = g->stream;
     switch (kind) {
     case Kind::SparsePush: launch_adv_k<WT, DIR_PUSH, SRC_SPARSE, PRED_NONE>(kidx, grid, s, a); break;
-    case Kind::DenseAll: launch_adv_k<WT, DIR_PUSH, SRC_DENSE, PRED_STAMP>(kidx, grid, s, a); break;
-    case Kind::DenseVerify: launch_adv_k<WT, DIR_PUSH, SRC_DENSE, PRED_REACHED>(kidx, grid, s, a); break;
     case Kind::DensePull: launch_adv_k<WT, DIR_PULL, SRC_DENSE, PRED_UNSETTLED>(kidx, grid, s, a); break;
     case Kind::SparseBfs: launch_adv_k<WT, DIR_BFS, SRC_SPARSE, PRED_NONE>(kidx, grid, s, a); break;
     }
@@ -640,15 +769,22 @@ void run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
                bool verify)
 {
     AdvArgs a = base_args<WT>(g, r, tick0);
-    if (verify) {
-        a.ent_v = g->nzo_v;
-        a.ent_scan = g->nzo_scan;
-        a.tile = g->nzo_tile[kidx];
-        a.n = g->n_nzo;
-        a.T = g->E;
-        a.adj = g->el_dst;
-        a.wts = g->el_w;
-        launch_advance<WT>(g, Kind::DenseVerify, kidx, a);
+    if (verify || mode == SSSP_MODE_ALL) {
+        // all-edge round streamed over the src-sorted edge list; the chunk
+        // bucket sets the grid (2, 4, 8 or 16 workgroups per CU)
+        hipStream_t s = g->stream;
+        const int64_t ng = g->E >> 2;
+        const int64_t nblocks = (int64_t)g->cu_count * (2 << kidx);
+        const int64_t per_block = (ng + nblocks - 1) / nblocks;
+        if (verify)
+            k_el_active<WT, PRED_REACHED><<<(unsigned)nblocks, kBlock, 0, s>>>(
+                g->el_src, g->el_dst, g->el_w, g->E, g->dist, g->stamp, a.tick_prev, a.tick_cur,
+                g->row_ptr, a.qnext, a.slot, per_block);
+        else
+            k_el_active<WT, PRED_STAMP><<<(unsigned)nblocks, kBlock, 0, s>>>(
+                g->el_src, g->el_dst, g->el_w, g->E, g->dist, g->stamp, a.tick_prev, a.tick_cur,
+                g->row_ptr, a.qnext, a.slot, per_block);
+        SSSP_LAUNCH_CHECK("k_el_active");
         return;
     }
     switch (mode) {
@@ -664,17 +800,6 @@ void run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
         a.adj = g->el_dst;
         a.wts = g->el_w;
         launch_advance<WT>(g, Kind::SparsePush, kidx, a);
-        break;
-    }
-    case SSSP_MODE_ALL: {
-        a.ent_v = g->nzo_v;
-        a.ent_scan = g->nzo_scan;
-        a.tile = g->nzo_tile[kidx];
-        a.n = g->n_nzo;
-        a.T = g->E;
-        a.adj = g->el_dst;
-        a.wts = g->el_w;
-        launch_advance<WT>(g, Kind::DenseAll, kidx, a);
         break;
     }
     case SSSP_MODE_PULL: {
@@ -715,17 +840,28 @@ void solve_plain(sssp_graph *g, sssp_stats *st)
     if (nblocks < 8) nblocks = 8;
     const int64_t per_block = (ng + nblocks - 1) / nblocks;
     int32_t pass = 0;
-    const int batch = 4;
+    // with trace_events every pass is bracketed by events (one pass per batch)
+    const int batch = g->trace_events ? 1 : 4;
     int32_t done_at = -1;
     while (done_at < 0) {
         for (int i = 0; i < batch; i++) {
+            if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
             k_el_pass<WT, uint32_t, 4><<<(unsigned)nblocks, kBlock, 0, s>>>(
                 g->el_src, g->el_dst, g->el_w, E, g->dist, g->ctrl->changed, pass + i, per_block);
             SSSP_LAUNCH_CHECK("k_el_pass");
+            if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r1, s));
         }
         SSSP_HIP(hipMemcpyAsync(g->ctrl_host->changed, g->ctrl->changed, sizeof(int32_t) * 64,
                                 hipMemcpyDeviceToHost, s));
         SSSP_HIP(hipStreamSynchronize(s));
+        if (g->trace_events) {
+            TraceRow tr{};
+            tr.mode = SSSP_MODE_ALL;
+            tr.frontier = g->V;
+            tr.frontier_edges = E;
+            SSSP_HIP(hipEventElapsedTime(&tr.ms, g->ev_r0, g->ev_r1));
+            g->trace.push_back(tr);
+        }
         for (int i = 0; i < batch; i++) {
             if (g->ctrl_host->changed[(pass + i) & 63] == 0) {
                 done_at = pass + i;

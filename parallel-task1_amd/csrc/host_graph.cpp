@@ -78,17 +78,15 @@ inline void rmat_edge(uint32_t *seed, int n, int32_t V, int a, int b, int c, int
     int x = V / 2, y = V / 2;
     const int a_end = a, b_end = a + b, c_end = a + b + c, d_end = a + b + c + d;
     for (int i = 1; i < n; i++) {
-        int step = 1 << (n - (i + 1));
-        int p = lcg_rand_r(seed) % 100;
-        if (p < a_end) {
-            x -= step; y -= step;
-        } else if (p < b_end) {
-            x -= step; y += step;
-        } else if (p < c_end) {
-            x += step; y -= step;
-        } else if (p < d_end) {
-            x += step; y += step;
-        }
+        const int step = 1 << (n - (i + 1));
+        const int p = lcg_rand_r(seed) % 100;
+        // quadrant a: (x-, y-), b: (x-, y+), c: (x+, y-), d: (x+, y+), else no
+        // move (graph_generation_API.hpp:446-461), evaluated without branches
+        const bool in_any = p < d_end;
+        const bool xplus = p >= b_end;
+        const bool yplus = (p >= a_end && p < b_end) || p >= c_end;
+        x += in_any ? (xplus ? step : -step) : 0;
+        y += in_any ? (yplus ? step : -step) : 0;
     }
     if (lcg_rand_r(seed) % 2 == 0) x--;
     if (lcg_rand_r(seed) % 2 == 0) y--;
@@ -291,10 +289,18 @@ int sssp_pick_sources(const int32_t *src, int64_t edges, int32_t vertices, uint6
     return sssp::guard([&] {
         SSSP_REQUIRE(vertices >= 1 && n >= 0 && (n == 0 || out), "pick_sources: bad arguments");
         std::vector<uint8_t> has_out((size_t)vertices, 0);
+        int64_t bad = 0;
+        uint8_t *ho = has_out.data();
+#pragma omp parallel for schedule(static) reduction(+ : bad)
         for (int64_t e = 0; e < edges; e++) {
-            SSSP_REQUIRE(src[e] >= 0 && src[e] < vertices, "pick_sources: src id out of range");
-            has_out[src[e]] = 1;
+            const int32_t s = src[e];
+            if (s < 0 || s >= vertices) {
+                bad++;
+                continue;
+            }
+            __atomic_store_n(&ho[s], (uint8_t)1, __ATOMIC_RELAXED);
         }
+        SSSP_REQUIRE(bad == 0, "pick_sources: src id out of range");
         int64_t cand = 0;
         for (int32_t v = 0; v < vertices; v++) cand += has_out[v];
         SSSP_REQUIRE(cand >= n, "pick_sources: fewer vertices with out-degree > 0 than requested");

@@ -160,6 +160,10 @@ __device__ inline int64_t block_excl_scan_i64(int64_t x, int64_t *total, int64_t
     return wpre + inc - x;
 }
 
+// gen.hip
+void rmat_device(int scale, int ef, int a, int b, int c, int d, uint32_t base_seed, int T,
+                 int32_t *src, int32_t *dst, float *w, hipStream_t s);
+
 // layout.hip
 void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s);
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,

@@ -149,6 +149,35 @@ __device__ __forceinline__ void wave_append(bool pred, int32_t item, int32_t *q,
     if (pred) q[base + __popcll(m & lanemask_lt())] = item;
 }
 
+// Wave-local staging of queue appends in LDS: lanes write their items into the
+// wave's own LDS buffer (no atomics), and the wave reserves space in the
+// global queue with one atomicAdd per flush. `cnt` is wave-uniform.
+__device__ __forceinline__ void stage_push(bool pred, int32_t item, int32_t *buf, int &cnt)
+{
+    const uint64_t m = __ballot(pred);
+    if (pred) buf[cnt + __popcll(m & lanemask_lt())] = item;
+    cnt += __popcll(m);
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void stage_flush(int32_t *buf, int &cnt, int32_t *q, int32_t *len)
+{
+    if (cnt == 0) return;
+    wave_lds_sync();
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(len, cnt);
+    base = __shfl(base, 0);
+    for (int i = __lane_id(); i < cnt; i += 64) q[base + i] = buf[i];
+    wave_lds_sync();
+    cnt = 0;
+}
+
 // Wave sum of a 64-bit value, result valid in every lane.
 __device__ __forceinline__ long long wave_sum64(long long x)
 {

@@ -66,6 +66,8 @@ _SIGS = {
     "sssp_device_count": (I32, [PI32]),
     "sssp_device_info": (I32, [I32, C.c_char_p, I32, PI32, PI64, PI32]),
     "sssp_graph_create": (I32, [I32, P, P, P, I32, I64, I32, U32, C.POINTER(P)]),
+    "sssp_graph_create_rmat": (I32, [I32, I32, I32, I32, I32, I32, U32, I32, I32, U32, P, P, P,
+                                     C.POINTER(P)]),
     "sssp_graph_destroy": (I32, [P]),
     "sssp_graph_info": (I32, [P, PI32, PI64, PI32, PI64]),
     "sssp_graph_download_layout": (I32, [P, P, P, P, P, P, P, P]),
@@ -226,6 +228,26 @@ def features(fv, fe, unsettled, rnd, mean_deg, V, E):
 # --------------------------------------------------------------------------
 class DeviceGraph:
     """A graph uploaded once to HBM (edge list sorted by src, CSR, CSC)."""
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a=57, b=19, c=19, d=5, seed=1, threads=64, device=0,
+             csc=True, download_edges=False):
+        """Generates the R-MAT graph on the device (bit-identical to rmat())
+        and builds it there; optionally returns the generated edge list."""
+        self = cls.__new__(cls)
+        E = (1 << scale) * edge_factor
+        arrs = (np.empty(E, np.int32), np.empty(E, np.int32), np.empty(E, np.float32)) \
+            if download_edges else (None, None, None)
+        h = C.c_void_p()
+        _check(lib().sssp_graph_create_rmat(scale, edge_factor, a, b, c, d, seed & 0xFFFFFFFF,
+                                            threads, device, 0 if csc else NO_CSC,
+                                            _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]),
+                                            C.byref(h)))
+        self._h = h
+        self.device = device
+        self.wtype, self.wdtype = W_F32, np.float32
+        self.V, self.E = 1 << scale, E
+        return (self, arrs) if download_edges else self
 
     def __init__(self, src, dst, w, vertices: int, device: int = 0, csc: bool = True):
         w = np.ascontiguousarray(w)

@@ -135,3 +135,20 @@ def test_verify_pass_and_trace(gpu):
         assert tr["mode"][-1] == S.MODE_VERIFY
         assert len(tr["mode"]) == st["iterations"]
         assert st["t_solve_ms"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scale,ef,seed,threads", [(12, 16, 3, 64), (16, 16, 0, 1), (18, 8, 11, 7)])
+def test_device_rmat_generator_matches_host(gpu, scale, ef, seed, threads):
+    s, d, w = S.rmat(scale, ef, seed=seed, threads=threads)
+    g, (ds, dd, dw) = S.DeviceGraph.rmat(scale, ef, seed=seed, threads=threads, download_edges=True)
+    try:
+        assert np.array_equal(ds, s) and np.array_equal(dd, d)
+        assert np.array_equal(bits(dw), bits(w))
+        L = g.layout()
+    finally:
+        g.close()
+    with S.DeviceGraph(s, d, w, 1 << scale) as h:
+        M = h.layout()
+    for k in ("row_ptr", "el_src", "el_dst", "col_ptr", "csc_src"):
+        assert np.array_equal(L[k], M[k]), k
