@@ -1,0 +1,139 @@
+"""Host-side tests without a GPU: the C ABI library loads and exports every
+declared symbol, argument errors come back as status codes, the binary
+edge-list format is byte-compatible with the reference's GraphStorageAPI, and
+the Python mirror of the reference interface behaves like it."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import sssp_amd as S
+from conftest import ROOT, bits
+
+HEADER = os.path.join(ROOT, "include", "sssp_mi355x.h")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sssp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_symbols()
+    assert len(names) > 40
+    out = subprocess.run(["nm", "-D", "--defined-only", S.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (sssp_[a-z0-9_]+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(lib, n)
+    # the Python binding declares a signature for each of them
+    assert set(names) <= set(S.exported_symbols())
+
+
+def test_argument_errors_are_status_codes(lib):
+    import ctypes as C
+    h = C.c_void_p()
+    rc = lib.sssp_graph_create(7, None, None, None, 4, 0, 0, 0, C.byref(h))
+    assert rc == S.ERR_ARG and b"weight type" in lib.sssp_last_error()
+    rc = lib.sssp_graph_create(S.W_F32, None, None, None, 0, 0, 0, 0, C.byref(h))
+    assert rc == S.ERR_ARG and b"vertex" in lib.sssp_last_error()
+    rc = lib.sssp_graph_create(S.W_F32, None, None, None, 4, 3, 0, 0, C.byref(h))
+    assert rc == S.ERR_ARG and b"null" in lib.sssp_last_error()
+    assert lib.sssp_solve(None, 0, 0, None, None) == S.ERR_ARG
+    assert lib.sssp_certify(None, 0, None, None, None) == S.ERR_ARG
+    assert lib.sssp_set_param(None, b"verify", 1.0) == S.ERR_ARG
+    assert lib.sssp_init_distances_f32(None, 10, 0) == S.ERR_ARG
+    assert lib.sssp_bellman_ford_f32(None, None, None, 10, 5, 0, None, None) == S.ERR_ARG
+    with pytest.raises(S.SSSPError):
+        S.rmat(0, 4)
+    with pytest.raises(S.SSSPError):
+        S.rmat(8, 4, a=60, b=30, c=20, d=5)
+    with pytest.raises(S.SSSPError):
+        S.er(10, 20, seed=1)  # average degree above V, as the reference throws
+
+
+def test_bin_writer_matches_reference_bytes(tmp_path):
+    ref = open(os.path.join(GOLD, "three_edges.bin"), "rb").read()
+    p = tmp_path / "g.bin"
+    S.save_bin(p, np.array([0, 1, 2]), np.array([1, 2, 0]),
+               np.array([0.5, 0.25, 1.0], np.float32), 3)
+    assert p.read_bytes() == ref
+    assert len(ref) == 1 + 4 + 8 + 3 * 12
+
+
+def test_bin_reader_reads_reference_file():
+    d, V, s, t, w = S.load_bin(os.path.join(GOLD, "three_edges.bin"))
+    assert d is True and V == 3
+    assert s.tolist() == [0, 1, 2] and t.tolist() == [1, 2, 0] and w.tolist() == [0.5, 0.25, 1.0]
+
+
+def test_bin_roundtrip_and_errors(tmp_path):
+    s, t, w = S.rmat(10, 4, seed=3, threads=2)
+    p = tmp_path / "r.bin"
+    S.save_bin(p, s, t, w, 1 << 10)
+    d, V, s2, t2, w2 = S.load_bin(p)
+    assert V == 1 << 10 and np.array_equal(s, s2) and np.array_equal(t, t2)
+    assert np.array_equal(bits(w), bits(w2))
+    # u32 weights
+    su, tu, wu = S.er(100, 4, seed=2, weight_mode=1)
+    pu = tmp_path / "u.bin"
+    S.save_bin(pu, su, tu, wu, 100)
+    _, _, _, _, wu2 = S.load_bin(pu, np.uint32)
+    assert np.array_equal(wu, wu2)
+    with pytest.raises(S.SSSPError) as e:
+        S.load_bin(tmp_path / "missing.bin")
+    assert e.value.code == S.ERR_IO
+    trunc = tmp_path / "t.bin"
+    trunc.write_bytes(p.read_bytes()[:100])
+    with pytest.raises(S.SSSPError) as e:
+        S.load_bin(trunc)
+    assert e.value.code == S.ERR_IO
+    hdr = tmp_path / "h.bin"
+    hdr.write_bytes(b"\x01\x02")
+    with pytest.raises(S.SSSPError) as e:
+        S.load_bin(hdr)
+    assert e.value.code == S.ERR_IO
+
+
+def test_storage_api_mirror(tmp_path):
+    g = S.GraphGenerationAPI.R_MAT_parallel(1 << 8, 4, 57, 19, 19, 5, 1, True)
+    p = tmp_path / "m.bin"
+    assert S.GraphStorageAPI.save_to_edges_list_bin_file(g, p)
+    h = S.GraphStorageAPI.load_from_edges_list_bin_file(p)
+    assert h.get_vertices_count() == 256 and h.get_edges_count() == 1024
+    with pytest.raises(S.SSSPError):
+        S.GraphStorageAPI.load_from_edges_list_bin_file(p, directed=False)
+    with pytest.raises(S.SSSPError):
+        S.GraphGenerationAPI.R_MAT_parallel(1 << 8, 4, 57, 19, 19, 5, 1, False)
+
+
+def test_pick_sources():
+    s, t, w = S.rmat(12, 8, seed=1, threads=8)
+    V = 1 << 12
+    a = S.pick_sources(s, V, 6, 16)
+    b = S.pick_sources(s, V, 6, 16)
+    assert np.array_equal(a, b) and len(set(a.tolist())) == 16
+    outdeg = np.bincount(s, minlength=V)
+    assert np.all(outdeg[a] > 0)
+    with pytest.raises(S.SSSPError):
+        S.pick_sources(np.zeros(3, np.int32), 4, 1, 2)
+
+
+def test_features_are_scale_free_and_bounded():
+    f = S.features(1, 10, 1.0, 0, 16, 1 << 20, 16 << 20)
+    g = S.features(32, 320, 1.0, 0, 16, 1 << 25, 16 << 25)
+    assert np.allclose(f, g, atol=1e-6)
+    for x in (f, S.features(0, 0, 0, 1000, 16, 10, 160), S.features(10, 160, 0.5, 3, 16, 10, 160)):
+        assert np.all((x >= 0) & (x <= 1))
+
+
+def test_python_mirror_fails_loudly_without_library(monkeypatch):
+    monkeypatch.setattr(S, "_lib", None)
+    monkeypatch.setattr(S, "LIB_PATH", "/nonexistent/libsssp_mi355x.so")
+    with pytest.raises(S.SSSPError):
+        S.lib()
