@@ -1,45 +1,568 @@
-// batch.hip -- batched solves (many sources on one graph, many small graphs).
+// batch.hip -- many SSSP solves in one launch.
+//
+// Policy 1 (workgroup per task): one persistent kernel; each workgroup takes
+// (graph range, source) tasks from an atomic counter -- largest tasks first --
+// and runs a complete frontier Bellman-Ford for its task with workgroup
+// barriers only (no grid barrier, so any occupancy is deadlock-free):
+//   frontier chunk of BS vertices -> degree scan in LDS -> the chunk's edges
+//   dealt over the BS lanes (binary search in LDS) -> check-then-atomicMin ->
+//   appends to the next frontier through an LDS counter, deduplicated by a
+//   per-workgroup stamp array. The launch (counter reset + persistent kernel)
+//   is captured once in a hipGraph and replayed.
+// Policy 2 (merged frontier): many disjoint graphs are one union graph, so all
+//   their sources share one full-chip frontier solve; many sources on one
+//   graph are solved one after another on the full chip.
+// The autotuner times block size x workgroups per CU for policy 1 and frontier
+// threshold x push chunk width for policy 2, and keeps the fastest.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+
 #include "sssp_graph.hpp"
+
+namespace sssp {
+
+struct BatchTask {
+    int64_t vlo;     // first vertex of the task's graph (global id)
+    int64_t doff;    // offset of the task's distance array in the batch output
+    int32_t nv;      // vertices of the task's graph
+    int32_t source;  // global id
+};
+
+namespace {
+
+// Block-wide exclusive scan, BS threads.
+template <int BS>
+__device__ int64_t bscan(int64_t x, int64_t *total, int64_t *s_warp)
+{
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    int64_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_warp[w] = inc;
+    __syncthreads();
+    int64_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; i++) {
+        if (i < w) pre += s_warp[i];
+        tot += s_warp[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - x;
+}
+
+__global__ void k_batch_reset(int32_t *counter)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counter[0] = 0;  // next task
+        counter[1] = 0;  // error flag
+    }
+}
+
+template <class WT, int BS>
+__global__ __launch_bounds__(BS) void k_wg_solve(const BatchTask *__restrict__ tasks, int32_t ntasks,
+                                                 int32_t *counter, const int64_t *__restrict__ row_ptr,
+                                                 const int32_t *__restrict__ adj,
+                                                 const uint32_t *__restrict__ wts, uint32_t *bdist,
+                                                 int32_t *scratch, int64_t maxv, int32_t *round_ctr)
+{
+    constexpr uint32_t INF = WT::INF;
+    __shared__ int s_task, s_n;
+    __shared__ int32_t s_v[BS];
+    __shared__ int64_t s_start[BS];
+    __shared__ int64_t s_off[BS + 1];
+    __shared__ uint32_t s_du[BS];
+    __shared__ int64_t s_warp[BS / 64];
+    const int tid = threadIdx.x;
+    int32_t *qa = scratch + (int64_t)blockIdx.x * 3 * maxv;
+    int32_t *qb = qa + maxv;
+    int32_t *stamp = qb + maxv;  // local index; rounds only ever increase
+    int32_t rt = round_ctr[blockIdx.x];
+    for (;;) {
+        if (tid == 0) s_task = atomicAdd(&counter[0], 1);
+        __syncthreads();
+        const int t = s_task;
+        __syncthreads();
+        if (t >= ntasks) break;
+        const BatchTask T = tasks[t];
+        uint32_t *dist = bdist + T.doff - T.vlo;  // indexed by global vertex id
+        for (int64_t v = tid; v < T.nv; v += BS) dist[T.vlo + v] = INF;
+        __syncthreads();
+        if (tid == 0) {
+            dist[T.source] = 0u;
+            qa[0] = T.source;
+        }
+        int32_t n = 1;
+        int32_t rounds = 0;
+        int32_t *cur = qa, *nxt = qb;
+        __syncthreads();
+        while (n > 0) {
+            if (++rounds > T.nv + 2) {  // cannot happen for w >= 0; bounded anyway
+                if (tid == 0) counter[1] = 1;
+                break;
+            }
+            rt++;
+            if (tid == 0) s_n = 0;
+            __syncthreads();
+            for (int32_t base = 0; base < n; base += BS) {
+                const int32_t i = base + tid;
+                int32_t u = 0;
+                int64_t a = 0, deg = 0;
+                uint32_t du = INF;
+                if (i < n) {
+                    u = cur[i];
+                    a = row_ptr[u];
+                    deg = row_ptr[u + 1] - a;
+                    du = dist[u];
+                }
+                s_v[tid] = u;
+                s_start[tid] = a;
+                s_du[tid] = du;
+                int64_t tot;
+                const int64_t off = bscan<BS>(deg, &tot, s_warp);
+                s_off[tid] = off;
+                if (tid == BS - 1) s_off[BS] = tot;
+                __syncthreads();
+                for (int64_t e = tid; e < tot; e += BS) {
+                    int lo = 0, hi = BS - 1;  // largest j with s_off[j] <= e
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_off[mid] <= e) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int64_t edge = s_start[lo] + (e - s_off[lo]);
+                    const int32_t v = adj[edge];
+                    const uint32_t cand = WT::add(s_du[lo], wts[edge]);
+                    if (cand < dist[v]) {
+                        const uint32_t old = atomicMin(&dist[v], cand);
+                        if (cand < old && atomicExch(&stamp[v - T.vlo], rt) != rt)
+                            nxt[atomicAdd(&s_n, 1)] = v;
+                    }
+                }
+                __syncthreads();
+            }
+            n = s_n;
+            int32_t *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+            __syncthreads();
+        }
+    }
+    if (tid == 0) round_ctr[blockIdx.x] = rt;
+}
+
+template <int BS>
+void launch_wg(int wtype, unsigned grid, hipStream_t s, const BatchTask *tasks, int32_t ntasks,
+               int32_t *counter, sssp_graph *g, uint32_t *bdist, int32_t *scratch, int64_t maxv)
+{
+    if (wtype == SSSP_W_F32)
+        k_wg_solve<WF32, BS><<<grid, BS, 0, s>>>(tasks, ntasks, counter, g->row_ptr, g->el_dst,
+                                                 g->el_w, bdist, scratch, maxv, g->bround);
+    else
+        k_wg_solve<WU32, BS><<<grid, BS, 0, s>>>(tasks, ntasks, counter, g->row_ptr, g->el_dst,
+                                                 g->el_w, bdist, scratch, maxv, g->bround);
+    SSSP_LAUNCH_CHECK("k_wg_solve");
+}
+
+struct Config {
+    int policy;      // 1 workgroup per task, 2 merged frontier / sequential
+    int bs;          // policy 1 block size
+    int wgpcu;       // policy 1 workgroups per CU
+    double frac;     // policy 2 push/all frontier threshold
+    int push_chunk;  // policy 2 sparse push chunk bucket
+};
+
+void ensure_batch_buffers(sssp_graph *g, int64_t dist_elems, int32_t ntasks, int32_t nwg,
+                          int64_t maxv)
+{
+    if (dist_elems > g->bdist_elems) {
+        dfree(g->bdist);
+        g->bdist = nullptr;
+        g->bdist = dmalloc<uint32_t>((size_t)dist_elems);
+        g->bdist_elems = dist_elems;
+    }
+    if (ntasks > g->btask_cap) {
+        dfree(g->btasks);
+        g->btasks = nullptr;
+        g->btasks = dmalloc<BatchTask>((size_t)ntasks);
+        g->btask_cap = ntasks;
+    }
+    if (!g->bcounter) g->bcounter = dmalloc<int32_t>(2);
+    // per-workgroup scratch = [queue a | queue b | stamps], each `stride`
+    // entries; the stride only grows, so stamp regions never move onto old
+    // queue contents
+    if (nwg > 0 && (maxv > g->bscratch_stride || nwg > g->bround_cap)) {
+        const int64_t stride = std::max<int64_t>(maxv, g->bscratch_stride);
+        const int32_t cap = std::max<int32_t>(nwg, g->bround_cap);
+        const int64_t need = (int64_t)cap * 3 * stride;
+        dfree(g->bscratch);
+        dfree(g->bround);
+        g->bscratch = nullptr;
+        g->bround = nullptr;
+        g->bscratch = dmalloc<int32_t>((size_t)need);
+        g->bscratch_elems = need;
+        g->bscratch_stride = stride;
+        g->bround = dmalloc<int32_t>((size_t)cap);
+        g->bround_cap = cap;
+        // stamps start below every round number the kernel will use
+        SSSP_HIP(hipMemsetAsync(g->bscratch, 0, 4 * (size_t)need, g->stream));
+        SSSP_HIP(hipMemsetAsync(g->bround, 0, 4 * (size_t)cap, g->stream));
+    }
+}
+
+// Policy 1: tasks already uploaded; captures (counter reset, persistent
+// kernel) in a hipGraph keyed on the launch shape and replays it.
+double run_wg(sssp_graph *g, int32_t ntasks, int64_t maxv, int bs, int wgpcu)
+{
+    hipStream_t s = g->stream;
+    if (maxv > g->bscratch_stride) throw Error{SSSP_ERR_INTERNAL, "batch scratch too small"};
+    maxv = g->bscratch_stride;
+    int32_t nwg = g->cu_count * wgpcu;
+    if (nwg > ntasks) nwg = ntasks;
+    if (nwg < 1) nwg = 1;
+    const int64_t key[8] = {ntasks, maxv, bs, nwg, (int64_t)(uintptr_t)g->bdist,
+                            (int64_t)(uintptr_t)g->bscratch, (int64_t)(uintptr_t)g->btasks,
+                            (int64_t)(uintptr_t)g->bround};
+    if (!g->batch_exec || std::memcmp(key, g->batch_key, sizeof(key)) != 0) {
+        if (g->batch_exec) {
+            (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
+            g->batch_exec = nullptr;
+        }
+        SSSP_HIP(hipStreamSynchronize(s));
+        hipGraph_t graph;
+        SSSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        k_batch_reset<<<1, 64, 0, s>>>(g->bcounter);
+        const BatchTask *tasks = static_cast<const BatchTask *>(g->btasks);
+        switch (bs) {
+        case 256: launch_wg<256>(g->wtype, nwg, s, tasks, ntasks, g->bcounter, g, g->bdist, g->bscratch, maxv); break;
+        case 512: launch_wg<512>(g->wtype, nwg, s, tasks, ntasks, g->bcounter, g, g->bdist, g->bscratch, maxv); break;
+        default: launch_wg<1024>(g->wtype, nwg, s, tasks, ntasks, g->bcounter, g, g->bdist, g->bscratch, maxv); break;
+        }
+        SSSP_HIP(hipStreamEndCapture(s, &graph));
+        hipGraphExec_t exec;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        SSSP_HIP(e);
+        g->batch_exec = exec;
+        std::memcpy(g->batch_key, key, sizeof(key));
+    }
+    SSSP_HIP(hipEventRecord(g->ev_a, s));
+    SSSP_HIP(hipGraphLaunch((hipGraphExec_t)g->batch_exec, s));
+    SSSP_HIP(hipEventRecord(g->ev_b, s));
+    SSSP_HIP(hipEventSynchronize(g->ev_b));
+    float ms = 0.f;
+    SSSP_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
+    int32_t flags[2] = {0, 0};
+    SSSP_HIP(hipMemcpy(flags, g->bcounter, sizeof(flags), hipMemcpyDeviceToHost));
+    if (flags[1]) throw Error{SSSP_ERR_INTERNAL, "batched solve exceeded its round bound"};
+    return ms;
+}
+
+void upload_tasks(sssp_graph *g, std::vector<BatchTask> &tasks, const std::vector<int64_t> &work)
+{
+    // largest tasks first (longest-processing-time order)
+    std::vector<size_t> order(tasks.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a] > work[b]; });
+    std::vector<BatchTask> sorted(tasks.size());
+    for (size_t i = 0; i < order.size(); i++) sorted[i] = tasks[order[i]];
+    SSSP_HIP(hipMemcpyAsync(g->btasks, sorted.data(), sizeof(BatchTask) * sorted.size(),
+                            hipMemcpyHostToDevice, g->stream));
+}
+
+// ---- many sources on one graph ----------------------------------------------
+double batch_sources(sssp_graph *g, const int32_t *sources, int32_t n, const Config &c)
+{
+    const int64_t V = g->V;
+    if (c.policy == 1) {
+        int32_t nwg = std::min<int32_t>(n, g->cu_count * c.wgpcu);
+        ensure_batch_buffers(g, (int64_t)n * V, n, nwg, V);
+        std::vector<BatchTask> tasks((size_t)n);
+        std::vector<int64_t> work((size_t)n, 1);
+        for (int32_t i = 0; i < n; i++) tasks[(size_t)i] = BatchTask{0, (int64_t)i * V, (int32_t)V, sources[i]};
+        upload_tasks(g, tasks, work);
+        // sources sorted by work are all equal here: order is preserved
+        return run_wg(g, n, V, c.bs, c.wgpcu);
+    }
+    ensure_batch_buffers(g, (int64_t)n * V, 0, 0, 0);
+    const double saved = g->push_edge_frac;
+    const int saved_chunk = g->push_chunk;
+    g->push_edge_frac = c.frac;
+    g->push_chunk = c.push_chunk;
+    double total = 0;
+    for (int32_t i = 0; i < n; i++) {
+        sssp_stats st{};
+        solve(g, &sources[i], 1, SSSP_SCHED_HEURISTIC, &st);
+        SSSP_HIP(hipMemcpyAsync(g->bdist + (int64_t)i * V, g->dist, 4 * (size_t)V,
+                                hipMemcpyDeviceToDevice, g->stream));
+        total += st.t_solve_ms;
+    }
+    SSSP_HIP(hipStreamSynchronize(g->stream));
+    g->push_edge_frac = saved;
+    g->push_chunk = saved_chunk;
+    return total;
+}
+
+std::vector<Config> candidates()
+{
+    std::vector<Config> c;
+    for (int bs : {256, 512, 1024})
+        for (int w : {1, 2, 4}) c.push_back(Config{1, bs, w, 0.05, 1});
+    for (double f : {0.02, 0.05, 0.15})
+        for (int pc : {1, 2}) c.push_back(Config{2, 256, 2, f, pc});
+    return c;
+}
+
+std::string describe(const Config &c)
+{
+    char buf[160];
+    if (c.policy == 1)
+        std::snprintf(buf, sizeof(buf), "policy=workgroup_per_task block=%d wg_per_cu=%d", c.bs, c.wgpcu);
+    else
+        std::snprintf(buf, sizeof(buf), "policy=merged_frontier push_edge_frac=%.3f push_chunk=%d", c.frac,
+                      c.push_chunk);
+    return buf;
+}
+
+template <class Run>
+Config autotune(Run &&run, const char *file, double *best_ms, const char *what)
+{
+    std::vector<Config> cs = candidates();
+    std::vector<double> t(cs.size(), 1e300);
+    for (size_t i = 0; i < cs.size(); i++) {
+        (void)run(cs[i]);  // warm-up (and graph capture)
+        t[i] = std::min(run(cs[i]), run(cs[i]));
+    }
+    const size_t b = std::min_element(t.begin(), t.end()) - t.begin();
+    if (file) {
+        FILE *f = std::fopen(file, "w");
+        if (!f) throw Error{SSSP_ERR_IO, std::string("cannot open tuning file ") + file};
+        std::fprintf(f, "# autotune %s: min of 2 timed runs after 1 warm-up, device ms\n", what);
+        for (size_t i = 0; i < cs.size(); i++)
+            std::fprintf(f, "%s ms=%.4f%s\n", describe(cs[i]).c_str(), t[i], i == b ? " best" : "");
+        std::fclose(f);
+    }
+    if (best_ms) *best_ms = t[b];
+    return cs[b];
+}
+
+Config default_config(sssp_graph *g, int32_t n)
+{
+    if (g->batch_policy)
+        return Config{g->batch_policy, g->batch_bs, g->batch_wgpcu, g->push_edge_frac, g->push_chunk};
+    if (n >= g->cu_count / 4) return Config{1, 256, 2, 0.05, 1};
+    return Config{2, 256, 2, g->push_edge_frac, g->push_chunk};
+}
+
+void fill_stats(sssp_stats *st, double ms, double edges)
+{
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    st->t_solve_ms = ms;
+    st->gteps = ms > 0 ? edges / (ms * 1e-3) / 1e9 : 0.0;
+}
+
+}  // namespace
+}  // namespace sssp
 
 using namespace sssp;
 
 struct sssp_pack {
-    int device = 0;
+    sssp_graph *g = nullptr;  // the disjoint union of all graphs
+    int32_t n = 0;
+    std::vector<int64_t> voff;  // n+1 vertex offsets
+    std::vector<int64_t> eoff;  // n+1 edge offsets
+    int policy = 0, bs = 256, wgpcu = 2;
+    double frac = 0.05;
+    int push_chunk = 1;
 };
+
+namespace sssp {
+namespace {
+
+double pack_run(sssp_pack *p, const int32_t *local_sources, const Config &c)
+{
+    sssp_graph *g = p->g;
+    const int32_t n = p->n;
+    if (c.policy == 1) {
+        int64_t maxv = 0;
+        for (int32_t k = 0; k < n; k++) maxv = std::max<int64_t>(maxv, p->voff[k + 1] - p->voff[k]);
+        const int32_t nwg = std::min<int32_t>(n, g->cu_count * c.wgpcu);
+        ensure_batch_buffers(g, g->V, n, nwg, maxv);
+        std::vector<BatchTask> tasks((size_t)n);
+        std::vector<int64_t> work((size_t)n);
+        for (int32_t k = 0; k < n; k++) {
+            tasks[(size_t)k] = BatchTask{p->voff[k], p->voff[k], (int32_t)(p->voff[k + 1] - p->voff[k]),
+                                         (int32_t)(p->voff[k] + local_sources[k])};
+            work[(size_t)k] = p->eoff[k + 1] - p->eoff[k];
+        }
+        upload_tasks(g, tasks, work);
+        return run_wg(g, n, maxv, c.bs, c.wgpcu);
+    }
+    std::vector<int32_t> gs((size_t)n);
+    for (int32_t k = 0; k < n; k++) gs[(size_t)k] = (int32_t)(p->voff[k] + local_sources[k]);
+    const double saved = g->push_edge_frac;
+    const int saved_chunk = g->push_chunk;
+    g->push_edge_frac = c.frac;
+    g->push_chunk = c.push_chunk;
+    sssp_stats st{};
+    solve(g, gs.data(), n, SSSP_SCHED_HEURISTIC, &st);
+    g->push_edge_frac = saved;
+    g->push_chunk = saved_chunk;
+    ensure_batch_buffers(g, g->V, 0, 0, 0);
+    SSSP_HIP(hipMemcpyAsync(g->bdist, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToDevice, g->stream));
+    SSSP_HIP(hipStreamSynchronize(g->stream));
+    return st.t_solve_ms;
+}
+
+void check_pack_sources(const sssp_pack *p, const int32_t *sources)
+{
+    SSSP_REQUIRE(sources, "null sources");
+    for (int32_t k = 0; k < p->n; k++)
+        SSSP_REQUIRE(sources[k] >= 0 && sources[k] < p->voff[k + 1] - p->voff[k],
+                     "pack source out of range for graph " + std::to_string(k));
+}
+
+}  // namespace
+}  // namespace sssp
 
 extern "C" {
 
 int sssp_solve_batch(sssp_graph *g, const int32_t *sources, int32_t n, int32_t policy,
                      void *dist_out, sssp_stats *stats)
 {
-    return fail(SSSP_ERR_INTERNAL, "batched solve not built yet");
+    return guard([&] {
+        SSSP_REQUIRE(g && sources && n >= 1, "bad arguments");
+        SSSP_REQUIRE(policy >= 0 && policy <= 2, "policy must be 0, 1 or 2");
+        for (int32_t i = 0; i < n; i++)
+            SSSP_REQUIRE(sources[i] >= 0 && sources[i] < g->V, "source vertex out of range");
+        SSSP_HIP(hipSetDevice(g->device));
+        Config c = default_config(g, n);
+        if (policy != 0 && policy != c.policy) c = (policy == 1) ? Config{1, 256, 2, 0.05, 1}
+                                                                 : Config{2, 256, 2, g->push_edge_frac, g->push_chunk};
+        const double ms = batch_sources(g, sources, n, c);
+        if (dist_out)
+            SSSP_HIP(hipMemcpy(dist_out, g->bdist, 4 * (size_t)n * (size_t)g->V, hipMemcpyDeviceToHost));
+        fill_stats(stats, ms, (double)g->E * n);
+        if (stats) stats->iterations = c.policy;
+    });
+}
+
+int sssp_autotune_batch(sssp_graph *g, const int32_t *sources, int32_t n, const char *tuning_file,
+                        double *best_ms)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && sources && n >= 1, "bad arguments");
+        for (int32_t i = 0; i < n; i++)
+            SSSP_REQUIRE(sources[i] >= 0 && sources[i] < g->V, "source vertex out of range");
+        SSSP_HIP(hipSetDevice(g->device));
+        const Config b = autotune([&](const Config &c) { return batch_sources(g, sources, n, c); },
+                                  tuning_file, best_ms, "batch of sources");
+        g->batch_policy = b.policy;
+        g->batch_bs = b.bs;
+        g->batch_wgpcu = b.wgpcu;
+        if (b.policy == 2) {
+            g->push_edge_frac = b.frac;
+            g->push_chunk = b.push_chunk;
+        }
+    });
 }
 
 int sssp_pack_create(const int32_t *src, const int32_t *dst, const float *weights,
                      const int64_t *edge_offsets, const int32_t *vertex_counts, int32_t n_graphs,
                      int32_t device, sssp_pack **out)
 {
-    return fail(SSSP_ERR_INTERNAL, "pack not built yet");
+    sssp_pack *p = nullptr;
+    int rc = guard([&] {
+        SSSP_REQUIRE(out && edge_offsets && vertex_counts && n_graphs >= 1, "bad arguments");
+        *out = nullptr;
+        p = new sssp_pack();
+        p->n = n_graphs;
+        p->voff.assign((size_t)n_graphs + 1, 0);
+        p->eoff.assign(edge_offsets, edge_offsets + n_graphs + 1);
+        SSSP_REQUIRE(p->eoff[0] == 0, "edge_offsets must start at 0");
+        for (int32_t k = 0; k < n_graphs; k++) {
+            SSSP_REQUIRE(vertex_counts[k] >= 1, "every graph needs a vertex");
+            SSSP_REQUIRE(p->eoff[k + 1] >= p->eoff[k], "edge_offsets must not decrease");
+            p->voff[k + 1] = p->voff[k] + vertex_counts[k];
+        }
+        const int64_t V = p->voff[n_graphs], E = p->eoff[n_graphs];
+        SSSP_REQUIRE(V < ((int64_t)1 << 31) - 1, "packed vertex count must fit int32");
+        SSSP_REQUIRE(E == 0 || (src && dst && weights), "null edge array");
+        std::vector<int32_t> us((size_t)E), ud((size_t)E);
+        int64_t bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : bad)
+        for (int32_t k = 0; k < n_graphs; k++) {
+            const int32_t nv = vertex_counts[k];
+            const int32_t o = (int32_t)p->voff[k];
+            for (int64_t e = p->eoff[k]; e < p->eoff[k + 1]; e++) {
+                if (src[e] < 0 || src[e] >= nv || dst[e] < 0 || dst[e] >= nv) bad++;
+                us[(size_t)e] = src[e] + o;
+                ud[(size_t)e] = dst[e] + o;
+            }
+        }
+        SSSP_REQUIRE(bad == 0, "pack: edge endpoint outside its graph");
+        const int rc2 = sssp_graph_create(SSSP_W_F32, us.data(), ud.data(), weights, (int32_t)V, E,
+                                          device, 0, &p->g);
+        if (rc2 != SSSP_OK) throw Error{rc2, sssp_last_error()};
+        *out = p;
+    });
+    if (rc != SSSP_OK && p) {
+        if (p->g) sssp_graph_destroy(p->g);
+        delete p;
+    }
+    return rc;
 }
 
-int sssp_pack_destroy(sssp_pack *p) { return fail(SSSP_ERR_INTERNAL, "pack not built yet"); }
+int sssp_pack_destroy(sssp_pack *p)
+{
+    return guard([&] {
+        SSSP_REQUIRE(p, "null pack");
+        if (p->g) sssp_graph_destroy(p->g);
+        delete p;
+    });
+}
 
 int sssp_pack_solve(sssp_pack *p, const int32_t *sources, int32_t policy, float *dist_out,
                     sssp_stats *stats)
 {
-    return fail(SSSP_ERR_INTERNAL, "pack not built yet");
-}
-
-int sssp_autotune_batch(sssp_graph *g, const int32_t *sources, int32_t n, const char *tuning_file,
-                        double *best_ms)
-{
-    return fail(SSSP_ERR_INTERNAL, "autotune not built yet");
+    return guard([&] {
+        SSSP_REQUIRE(p && p->g, "null pack");
+        SSSP_REQUIRE(policy >= 0 && policy <= 2, "policy must be 0, 1 or 2");
+        check_pack_sources(p, sources);
+        SSSP_HIP(hipSetDevice(p->g->device));
+        Config c{p->policy ? p->policy : 1, p->bs, p->wgpcu, p->frac, p->push_chunk};
+        if (policy == 1) c = Config{1, p->policy == 1 ? p->bs : 256, p->policy == 1 ? p->wgpcu : 2, 0.05, 1};
+        if (policy == 2) c = Config{2, 256, 2, p->policy == 2 ? p->frac : 0.05, p->policy == 2 ? p->push_chunk : 1};
+        const double ms = pack_run(p, sources, c);
+        if (dist_out)
+            SSSP_HIP(hipMemcpy(dist_out, p->g->bdist, 4 * (size_t)p->g->V, hipMemcpyDeviceToHost));
+        fill_stats(stats, ms, (double)p->g->E);
+        if (stats) stats->iterations = c.policy;
+    });
 }
 
 int sssp_autotune_pack(sssp_pack *p, const int32_t *sources, const char *tuning_file,
                        double *best_ms)
 {
-    return fail(SSSP_ERR_INTERNAL, "autotune not built yet");
+    return guard([&] {
+        SSSP_REQUIRE(p && p->g, "null pack");
+        check_pack_sources(p, sources);
+        SSSP_HIP(hipSetDevice(p->g->device));
+        const Config b = autotune([&](const Config &c) { return pack_run(p, sources, c); },
+                                  tuning_file, best_ms, "pack of graphs");
+        p->policy = b.policy;
+        p->bs = b.bs;
+        p->wgpcu = b.wgpcu;
+        p->frac = b.frac;
+        p->push_chunk = b.push_chunk;
+    });
 }
 
 }  // extern "C"

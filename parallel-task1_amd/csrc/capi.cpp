@@ -41,7 +41,9 @@ void graph_free(sssp_graph *g)
     void *ptrs[] = {g->row_ptr, g->el_src, g->el_dst, g->el_w, g->col_ptr, g->csc_src, g->csc_w,
                     g->nzo_v, g->nzo_scan, g->nzi_v, g->nzi_scan, g->sample_v, g->dist, g->stamp,
                     g->queue[0], g->queue[1], g->lb_v, g->lb_start, g->lb_scan, g->lb_tile,
-                    g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch};
+                    g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
+                    g->src_list, g->btasks, g->bcounter, g->bround};
+    if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {
         dfree(g->nzo_tile[c]);
@@ -299,7 +301,7 @@ int sssp_solve(sssp_graph *g, int32_t source, int32_t schedule, void *dist_out, 
         sssp_stats st;
         std::memset(&st, 0, sizeof(st));
         g->have_result = false;
-        solve(g, source, schedule, &st);
+        solve(g, &source, 1, schedule, &st);
         g->last_source = source;
         g->last_sched = schedule;
         g->have_result = true;
@@ -344,7 +346,7 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
         SSSP_REQUIRE(g->have_result, "no solve result on this graph");
         SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
         set_device(g->device);
-        certify(g, source, counts);
+        certify(g, &source, 1, counts);
         if (parent_out)
             SSSP_HIP(hipMemcpy(parent_out, g->parent, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
         if (valid) *valid = (counts[0] == 0 && counts[1] == 0 && counts[2] == 0) ? 1 : 0;

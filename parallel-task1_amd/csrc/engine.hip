@@ -438,23 +438,43 @@ __device__ void zero_slot(RoundSlot *s, uint32_t minw)
     s->sample_unsettled_in = 0;
 }
 
+// source < 0: only the control block is reset (k_init_sources seeds the queue)
 __global__ void k_init_solve(Ctrl *c, uint32_t *dist, int32_t *stamp, int32_t *q0,
                              const int64_t *row_ptr, int32_t source, int32_t tick0)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    dist[source] = 0u;
-    const int64_t deg = row_ptr[source + 1] - row_ptr[source];
     zero_slot(&c->slot[0], 0u);
     zero_slot(&c->slot[1], 0xFFFFFFFFu);
+    for (int i = 0; i < 64; i++) c->changed[i] = 0;
+    for (int i = 0; i < 8; i++) c->cert[i] = 0;
+    c->err = 0;
+    if (source < 0) return;
+    dist[source] = 0u;
+    const int64_t deg = row_ptr[source + 1] - row_ptr[source];
     c->slot[0].qlen = deg > 0 ? 1 : 0;
     c->slot[0].qedges = deg;
     if (deg > 0) {
         q0[0] = source;
         stamp[source] = tick0;
     }
-    for (int i = 0; i < 64; i++) c->changed[i] = 0;
-    for (int i = 0; i < 8; i++) c->cert[i] = 0;
-    c->err = 0;
+}
+
+// Several sources at distance 0 (a batch of disjoint graphs solved as one
+// union graph): every source with out-edges starts in the first frontier.
+__global__ __launch_bounds__(kBlock) void k_init_sources(Ctrl *c, uint32_t *dist, int32_t *stamp,
+                                                         int32_t *q0, const int64_t *row_ptr,
+                                                         const int32_t *srcs, int32_t n,
+                                                         int32_t tick0)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = srcs[i];
+    dist[s] = 0u;
+    const int64_t deg = row_ptr[s + 1] - row_ptr[s];
+    if (deg > 0 && atomicExch(&stamp[s], tick0) != tick0) {
+        q0[atomicAdd(&c->slot[0].qlen, 1)] = s;
+        atomicAdd((unsigned long long *)&c->slot[0].qedges, (unsigned long long)deg);
+    }
 }
 
 // Zeroes the slot the coming round fills and samples the unsettled fraction
@@ -651,23 +671,35 @@ __global__ __launch_bounds__(kBlock) void k_cert_edges(const int32_t *src, const
     if (__lane_id() == 0 && n) atomicAdd((unsigned long long *)viol, (unsigned long long)n);
 }
 
-__global__ void k_cert_root(Ctrl *c, int32_t *parent, int32_t *q0, const int64_t *row_ptr,
-                            const uint32_t *dist, int32_t source)
+__global__ void k_cert_root(Ctrl *c)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    parent[source] = source;  // visited marker during the BFS
-    const int64_t deg = row_ptr[source + 1] - row_ptr[source];
     zero_slot(&c->slot[0], 0u);
     zero_slot(&c->slot[1], 0xFFFFFFFFu);
-    c->slot[0].qlen = deg > 0 ? 1 : 0;
-    c->slot[0].qedges = deg;
-    if (deg > 0) q0[0] = source;
-    c->cert[1] = dist[source] != 0u ? 1 : 0;
+}
+
+// BFS roots of the tight-edge tree: every source (its own parent marks it
+// visited during the BFS); counts sources whose distance is not 0.
+__global__ __launch_bounds__(kBlock) void k_cert_roots(Ctrl *c, int32_t *parent, int32_t *q0,
+                                                       const int64_t *row_ptr, const uint32_t *dist,
+                                                       const int32_t *srcs, int32_t one_src,
+                                                       int32_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = srcs ? srcs[i] : one_src;
+    if (atomicCAS(&parent[s], -1, s) != -1) return;
+    if (dist[s] != 0u) atomicAdd((unsigned long long *)&c->cert[1], 1ull);
+    const int64_t deg = row_ptr[s + 1] - row_ptr[s];
+    if (deg > 0) {
+        q0[atomicAdd(&c->slot[0].qlen, 1)] = s;
+        atomicAdd((unsigned long long *)&c->slot[0].qedges, (unsigned long long)deg);
+    }
 }
 
 template <class WT>
 __global__ __launch_bounds__(kBlock) void k_cert_count(const uint32_t *dist, int32_t *parent,
-                                                       int32_t V, int32_t source, int64_t *cert)
+                                                       int32_t V, int64_t *cert)
 {
     int64_t reach = 0, untight = 0, tree = 0;
     for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
@@ -677,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_cert_count(const uint32_t *dist, int
         reach += fin;
         if (fin && p == -1) untight++;
         if (p != -1) tree++;
-        if (v == source) parent[v] = -1;
+        if (p == v) parent[v] = -1;  // a root: only sources are their own parent
     }
     reach = wave_sum64(reach);
     untight = wave_sum64(untight);
@@ -967,8 +999,19 @@ void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_
     st->reached = reached;
 }
 
+void upload_sources(sssp_graph *g, const int32_t *sources, int32_t n)
+{
+    if (n > g->src_cap) {
+        dfree(g->src_list);
+        g->src_list = nullptr;
+        g->src_list = dmalloc<int32_t>((size_t)n);
+        g->src_cap = n;
+    }
+    SSSP_HIP(hipMemcpyAsync(g->src_list, sources, 4 * (size_t)n, hipMemcpyHostToDevice, g->stream));
+}
+
 template <class WT>
-void solve_t(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
+void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp_stats *st)
 {
     hipStream_t s = g->stream;
     if (g->tick > (1 << 30) - g->max_rounds - 4) {
@@ -977,12 +1020,20 @@ void solve_t(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
         g->tick = 1;
     }
     const int32_t tick0 = g->tick;
+    const int32_t source = sources[0];
+    if (n_src > 1) upload_sources(g, sources, n_src);
     g->trace.clear();
     SSSP_HIP(hipEventRecord(g->ev_a, s));
     k_fill<uint32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->V, WT::INF);
     SSSP_LAUNCH_CHECK("k_fill dist");
-    k_init_solve<<<1, 64, 0, s>>>(g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, source, tick0);
+    k_init_solve<<<1, 64, 0, s>>>(g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr,
+                                  n_src > 1 ? -1 : source, tick0);
     SSSP_LAUNCH_CHECK("k_init_solve");
+    if (n_src > 1) {
+        k_init_sources<<<grid_for(n_src, 1 << 20), kBlock, 0, s>>>(
+            g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, g->src_list, n_src, tick0);
+        SSSP_LAUNCH_CHECK("k_init_sources");
+    }
     if (sched == SSSP_SCHED_PLAIN) {
         solve_plain<WT>(g, st);
         g->tick = tick0 + 1;
@@ -1003,8 +1054,9 @@ void solve_t(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
 }
 
 template <class WT>
-void certify_t(sssp_graph *g, int32_t source, int64_t *counts)
+void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *counts)
 {
+    if (n_src > 1) upload_sources(g, sources, n_src);
     hipStream_t s = g->stream;
     Ctrl *c = g->ctrl;
     k_fill<int64_t><<<1, 64, 0, s>>>(c->cert, 8, 0);
@@ -1012,8 +1064,11 @@ void certify_t(sssp_graph *g, int32_t source, int64_t *counts)
                                                        &c->cert[0]);
     SSSP_LAUNCH_CHECK("k_cert_edges");
     k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->parent, g->V, -1);
-    k_cert_root<<<1, 64, 0, s>>>(c, g->parent, g->queue[0], g->row_ptr, g->dist, source);
-    SSSP_LAUNCH_CHECK("k_cert_root");
+    k_cert_root<<<1, 64, 0, s>>>(c);
+    k_cert_roots<<<grid_for(n_src, 1 << 20), kBlock, 0, s>>>(c, g->parent, g->queue[0], g->row_ptr,
+                                                              g->dist, n_src > 1 ? g->src_list : nullptr,
+                                                              sources[0], n_src);
+    SSSP_LAUNCH_CHECK("k_cert_roots");
     for (int32_t r = 0;; r++) {
         if (r > g->V + 2) throw Error{SSSP_ERR_INTERNAL, "certificate BFS did not terminate"};
         k_round_begin<<<1, kBlock, 0, s>>>(c, r, g->sample_v, 0, g->dist, nullptr);
@@ -1033,7 +1088,7 @@ void certify_t(sssp_graph *g, int32_t source, int64_t *counts)
         a.wts = g->el_w;
         launch_advance<WT>(g, Kind::SparseBfs, 2, a);
     }
-    k_cert_count<WT><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->parent, g->V, source, c->cert);
+    k_cert_count<WT><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->parent, g->V, c->cert);
     SSSP_LAUNCH_CHECK("k_cert_count");
     SSSP_HIP(hipMemcpyAsync(counts, c->cert, sizeof(int64_t) * 8, hipMemcpyDeviceToHost, s));
     SSSP_HIP(hipStreamSynchronize(s));
@@ -1135,10 +1190,10 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
 
 }  // namespace
 
-void solve(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
+void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp_stats *st)
 {
-    if (g->wtype == SSSP_W_F32) solve_t<WF32>(g, source, sched, st);
-    else solve_t<WU32>(g, source, sched, st);
+    if (g->wtype == SSSP_W_F32) solve_t<WF32>(g, sources, n_src, sched, st);
+    else solve_t<WU32>(g, sources, n_src, sched, st);
 }
 
 void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
@@ -1148,10 +1203,10 @@ void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, f
     else collect_t<WU32>(g, source, cap, feats, times, rows);
 }
 
-void certify(sssp_graph *g, int32_t source, int64_t *counts)
+void certify(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *counts)
 {
-    if (g->wtype == SSSP_W_F32) certify_t<WF32>(g, source, counts);
-    else certify_t<WU32>(g, source, counts);
+    if (g->wtype == SSSP_W_F32) certify_t<WF32>(g, sources, n_src, counts);
+    else certify_t<WU32>(g, sources, n_src, counts);
 }
 
 // ---------------------------------------------------------------------------

@@ -130,8 +130,22 @@ struct sssp_graph {
     int64_t bdist_elems = 0;
     int32_t *bscratch = nullptr;
     int64_t bscratch_elems = 0;
+    int64_t bscratch_stride = 0;
 
     int64_t device_bytes = 0;
+
+    // multi-source solves: device copy of the source list
+    int32_t *src_list = nullptr;
+    int32_t src_cap = 0;
+    // batch policy chosen by the autotuner (0 = not tuned)
+    int batch_policy = 0, batch_bs = 256, batch_wgpcu = 2;
+    void *batch_exec = nullptr;  // hipGraphExec_t of the captured persistent launch
+    int64_t batch_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    void *btasks = nullptr;      // device BatchTask array
+    int32_t btask_cap = 0;
+    int32_t *bcounter = nullptr;  // task counter + error flag
+    int32_t *bround = nullptr;    // per-workgroup round counters
+    int32_t bround_cap = 0;
 };
 
 namespace sssp {
@@ -180,8 +194,8 @@ void ref_init(DT *d, int32_t V, int32_t source);
 template <class WT, class DT>
 void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int64_t E, DT *d,
             int32_t *passes_out);
-void solve(sssp_graph *g, int32_t source, int sched, sssp_stats *st);
-void certify(sssp_graph *g, int32_t source, int64_t *counts);
+void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp_stats *st);
+void certify(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *counts);
 void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
                      int32_t *rows);
 

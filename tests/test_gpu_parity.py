@@ -152,3 +152,84 @@ def test_device_rmat_generator_matches_host(gpu, scale, ef, seed, threads):
         M = h.layout()
     for k in ("row_ptr", "el_src", "el_dst", "col_ptr", "csc_src"):
         assert np.array_equal(L[k], M[k]), k
+
+
+# ---------------------------------------------------------------- batches
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy", [1, 2])
+def test_batch_sources_match_single_solves(gpu, oracle, policy):
+    src, dst, w = S.rmat(13, 16, seed=21, threads=16)
+    V = 1 << 13
+    sources = S.pick_sources(src, V, 21, 48)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        D, st = g.solve_batch(sources, policy=policy)
+        assert D.shape == (48, V) and st["t_solve_ms"] > 0
+        for i in (0, 7, 47):
+            ref, _ = oracle.bf(src, dst, w, V, int(sources[i]))
+            assert np.array_equal(bits(D[i]), bits(ref))
+        for i in range(48):
+            d, _ = g.solve(int(sources[i]), "push")
+            assert np.array_equal(bits(D[i]), bits(d)), i
+        # replay of the captured launch with other sources
+        s2 = sources[::-1].copy()
+        D2, _ = g.solve_batch(s2, policy=policy)
+        assert np.array_equal(bits(D2[0]), bits(D[-1]))
+
+
+@pytest.mark.gpu
+def test_batch_u32(gpu, oracle):
+    src, dst, w = S.er(1 << 12, 8, seed=4, weight_mode=1)
+    V = 1 << 12
+    with S.DeviceGraph(src, dst, w, V) as g:
+        D, _ = g.solve_batch(np.array([0, 5, 9], np.int32), policy=1)
+        for i, s in enumerate((0, 5, 9)):
+            assert np.array_equal(D[i], oracle.dijkstra(src, dst, w, V, s))
+
+
+def _pack_case(n, seed):
+    rng = np.random.default_rng(seed)
+    scales = rng.integers(4, 11, n).astype(np.int32)
+    s, d, w, offs = S.rmat_many(scales, 8, seed0=seed)
+    vcounts = (1 << scales).astype(np.int32)
+    srcs = np.array([int(s[offs[k]]) for k in range(n)], np.int32)  # a vertex with out-edges
+    return scales, s, d, w, offs, vcounts, srcs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy", [1, 2])
+def test_pack_matches_per_graph_oracle(gpu, oracle, policy):
+    scales, s, d, w, offs, vc, srcs = _pack_case(40, 3)
+    p = S.GraphPack(s, d, w, offs, vc)
+    try:
+        D, st = p.solve(srcs, policy=policy)
+    finally:
+        p.close()
+    voff = np.concatenate([[0], np.cumsum(vc)])
+    for k in range(40):
+        e0, e1 = offs[k], offs[k + 1]
+        ref, _ = oracle.bf(s[e0:e1], d[e0:e1], w[e0:e1], int(vc[k]), int(srcs[k]))
+        assert np.array_equal(bits(D[voff[k]:voff[k + 1]]), bits(ref)), k
+
+
+@pytest.mark.gpu
+def test_autotune_batch_and_pack(gpu, tmp_path):
+    src, dst, w = S.rmat(12, 8, seed=2, threads=8)
+    V = 1 << 12
+    sources = S.pick_sources(src, V, 2, 16)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        ref, _ = g.solve_batch(sources, policy=2)
+        best = g.autotune_batch(sources, tmp_path / "tune.txt")
+        assert best > 0
+        text = (tmp_path / "tune.txt").read_text()
+        assert text.count("best") == 1 and "workgroup_per_task" in text and "merged_frontier" in text
+        D, _ = g.solve_batch(sources, policy=0)
+        assert np.array_equal(bits(D), bits(ref))
+    scales, s, d, w2, offs, vc, srcs = _pack_case(24, 5)
+    p = S.GraphPack(s, d, w2, offs, vc)
+    try:
+        a, _ = p.solve(srcs, policy=2)
+        assert p.autotune(srcs, tmp_path / "pack.txt") > 0
+        b, _ = p.solve(srcs, policy=0)
+        assert np.array_equal(bits(a), bits(b))
+    finally:
+        p.close()
