@@ -1,0 +1,117 @@
+"""Trains the learned round scheduler on timings the solver itself measures.
+
+For directed R-MAT graphs of scales 18..22 (edge factor 16, a/b/c/d =
+57/19/19/5, generated on the device) and several sources each,
+sssp_collect_timings runs every relaxation round three ways (push, all-edge,
+pull) at each of the four chunk widths from the same saved state, times them
+with device events, and continues with the fastest. Each round becomes one
+training row: the 5 round features -> the fastest mode and, for that mode, the
+fastest chunk bucket. The network (5 -> 64 -> 64 -> 3 + 4) is trained on the
+GPU by the library's own forward/backward/Adam kernels, and the weights are
+written to models/scheduler_mlp.f32 (raw float32, the library's parameter
+order) with a JSON sidecar describing the run.
+
+    python parallel-task1_amd/train_scheduler.py [--scales 18 19 20 21 22]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import sssp_amd as S  # noqa: E402
+
+MODELS = os.path.join(HERE, "models")
+
+
+def collect(scales, sources_per_scale, edge_factor, seed0):
+    X, T, meta = [], [], []
+    for k, scale in enumerate(scales):
+        seed = seed0 + k
+        g = S.DeviceGraph.rmat(scale, edge_factor, seed=seed, threads=64)
+        L = g.layout(csc=False)
+        deg = np.diff(L["row_ptr"])
+        srcs = S.pick_sources(np.repeat(np.arange(g.V, dtype=np.int32), deg), g.V, seed,
+                              sources_per_scale)
+        for s in srcs:
+            t0 = time.perf_counter()
+            f, t = g.collect_timings(int(s))
+            X.append(f)
+            T.append(t)
+            meta.append({"scale": scale, "seed": seed, "source": int(s), "rounds": len(f),
+                         "seconds": round(time.perf_counter() - t0, 2)})
+            print(f"scale {scale} source {int(s)}: {len(f)} rounds", flush=True)
+        g.close()
+    return np.concatenate(X), np.concatenate(T), meta
+
+
+def labels(T):
+    """Fastest mode (over its best chunk) and that mode's fastest chunk."""
+    t = T.reshape(len(T), S.MLP_MODES, S.MLP_CHUNKS).astype(np.float64)
+    t = np.where(t < 0, np.inf, t)  # unavailable mode
+    best_per_mode = t.min(axis=2)
+    ym = best_per_mode.argmin(axis=1).astype(np.int32)
+    yc = t[np.arange(len(t)), ym].argmin(axis=1).astype(np.int32)
+    return ym, yc, t
+
+
+def regret(t, pm, pc):
+    """Total time of the chosen (mode, chunk) over the total of the best."""
+    chosen = t[np.arange(len(t)), pm, pc]
+    return float(chosen.sum() / t.reshape(len(t), -1).min(axis=1).sum())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scales", type=int, nargs="+", default=[18, 19, 20, 21, 22])
+    ap.add_argument("--sources", type=int, default=6)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--out", default=os.path.join(MODELS, "scheduler_mlp.f32"))
+    args = ap.parse_args()
+
+    X, T, meta = collect(args.scales, args.sources, args.edge_factor, args.seed)
+    ym, yc, t = labels(T)
+    n = len(X)
+    rng = np.random.default_rng(args.seed)
+    perm = rng.permutation(n)
+    nval = max(1, n // 5)
+    val, tr = perm[:nval], perm[nval:]
+
+    mlp = S.MLP(0, seed=args.seed)
+    losses = mlp.train(X[tr], ym[tr], yc[tr], epochs=args.epochs, batch=min(args.batch, len(tr)),
+                       lr=args.lr, seed=args.seed)
+    out = mlp.forward(X)
+    pm = out[:, :S.MLP_MODES].argmax(axis=1)
+    pc = out[:, S.MLP_MODES:].argmax(axis=1)
+    # the heuristic's choice on the same rows, for comparison (fixed chunks)
+    report = {
+        "rows": int(n), "train_rows": int(len(tr)), "val_rows": int(nval),
+        "scales": args.scales, "edge_factor": args.edge_factor, "sources_per_scale": args.sources,
+        "label_mode_counts": np.bincount(ym, minlength=S.MLP_MODES).tolist(),
+        "final_train_loss": float(losses[-1]) if len(losses) else None,
+        "val_mode_accuracy": float((pm[val] == ym[val]).mean()),
+        "val_chunk_accuracy": float((pc[val] == yc[val]).mean()),
+        "val_time_regret": regret(t[val], pm[val], pc[val]),
+        "train_time_regret": regret(t[tr], pm[tr], pc[tr]),
+        "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
+        "collection": meta,
+    }
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    mlp.params().astype(np.float32).tofile(args.out)
+    np.savez(os.path.splitext(args.out)[0] + "_data.npz", X=X, T=T)
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump(report, f, indent=1)
+    print(json.dumps({k: v for k, v in report.items() if k != "collection"}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

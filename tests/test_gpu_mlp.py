@@ -1,0 +1,127 @@
+"""The scheduler network's GPU kernels against the CPU oracle (oracle/mlp_ref.py),
+and what the trained network does inside the solver."""
+import os
+
+import numpy as np
+import pytest
+
+import mlp_ref as M
+import sssp_amd as S
+from conftest import ROOT, bits
+
+WEIGHTS = os.path.join(ROOT, "parallel-task1_amd", "models", "scheduler_mlp.f32")
+
+
+def _case(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, M.IN)).astype(np.float32)
+    ym = rng.integers(0, M.MODES, n).astype(np.int32)
+    yc = rng.integers(0, M.CHUNKS, n).astype(np.int32)
+    p = rng.normal(0, 0.3, M.NPARAM).astype(np.float32)
+    return p, x, ym, yc
+
+
+@pytest.mark.gpu
+def test_forward_matches_oracle(gpu):
+    p, x, _, _ = _case(37, 0)
+    m = S.MLP(0)
+    m.set_params(p)
+    assert np.array_equal(m.params(), p)
+    out = m.forward(x)
+    ref, _ = M.forward(p.astype(np.float64), x.astype(np.float64))
+    # fp32 dot products of length <= 64 against float64: relative 1e-5
+    assert np.allclose(out, ref, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_gradient_matches_oracle(gpu):
+    p, x, ym, yc = _case(64, 1)
+    m = S.MLP(0)
+    m.set_params(p)
+    loss, g = m.loss_grad(x, ym, yc)
+    rl, rg = M.loss_grad(p.astype(np.float64), x.astype(np.float64), ym, yc)
+    assert abs(loss - rl) <= 1e-5 * max(1.0, abs(rl))
+    scale = np.abs(rg).max()
+    assert np.allclose(g, rg, rtol=1e-4, atol=1e-5 * scale)
+
+
+@pytest.mark.gpu
+def test_adam_step_matches_oracle(gpu):
+    p, x, ym, yc = _case(32, 2)
+    m = S.MLP(0)
+    m.set_params(p)
+    m.train(x, ym, yc, epochs=1, batch=32, lr=1e-2, seed=0)  # one full-batch step
+    _, rg = M.loss_grad(p.astype(np.float64), x.astype(np.float64), ym, yc)
+    ref, _, _ = M.adam_step(p.astype(np.float64), rg, np.zeros(M.NPARAM), np.zeros(M.NPARAM), 1, 1e-2)
+    assert np.allclose(m.params(), ref, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_training_lowers_loss_on_a_learnable_rule(gpu):
+    rng = np.random.default_rng(3)
+    x = rng.random((512, M.IN)).astype(np.float32)
+    ym = (x[:, 1] > 0.5).astype(np.int32) + (x[:, 2] > 0.7).astype(np.int32)
+    yc = (x[:, 0] * 4).astype(np.int32).clip(0, 3)
+    m = S.MLP(0, seed=1)
+    l0, _ = m.loss_grad(x, ym, yc)
+    losses = m.train(x, ym, yc, epochs=150, batch=64, lr=5e-3, seed=1)
+    assert losses[-1] < 0.5 * l0
+    acc = (m.forward(x)[:, :3].argmax(1) == ym).mean()
+    assert acc > 0.85
+
+
+@pytest.mark.gpu
+def test_any_network_keeps_the_fixpoint(gpu, oracle):
+    """Whatever the network picks, distances are the Bellman-Ford fixpoint:
+    random and constant-output networks give bit-identical results."""
+    src, dst, w = S.rmat(13, 16, seed=12, threads=8)
+    V = 1 << 13
+    ref = oracle.dijkstra(src, dst, w, V, 0)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        for seed in range(3):
+            m = S.MLP(0, seed=seed)
+            g.set_mlp(m)
+            d, st = g.solve(0, "learned")
+            assert st["mlp_rounds"] > 0
+            assert np.array_equal(bits(d), bits(ref))
+        for mode in range(3):  # network that always answers `mode`
+            p = np.zeros(S.MLP_NPARAM, np.float32)
+            p[-S.MLP_OUT + mode] = 5.0
+            m = S.MLP(0)
+            m.set_params(p)
+            g.set_mlp(m)
+            d, st = g.solve(0, "learned")
+            assert np.array_equal(bits(d), bits(ref)), mode
+        g.set_mlp(None)
+
+
+@pytest.mark.gpu
+def test_trained_network_against_heuristic(gpu):
+    """Reports what the trained network gains over the hand-written rule on an
+    R-MAT graph of scale 21 (inside the training range but unseen seed), and
+    checks that it does not lose much; distances must be identical."""
+    if not os.path.exists(WEIGHTS):
+        pytest.skip("no trained weights (run parallel-task1_amd/train_scheduler.py)")
+    g = S.DeviceGraph.rmat(21, 16, seed=99, threads=64)
+    try:
+        L = g.layout(csc=False)
+        deg = np.diff(L["row_ptr"])
+        srcs = S.pick_sources(np.repeat(np.arange(g.V, dtype=np.int32), deg), g.V, 99, 4)
+        m = S.MLP(0)
+        m.set_params(np.fromfile(WEIGHTS, np.float32))
+        g.set_mlp(m)
+        th, tl = [], []
+        for s in srcs:
+            for _ in range(2):  # warm
+                dh, sh = g.solve(int(s), "heuristic")
+                dl, sl = g.solve(int(s), "learned")
+            assert np.array_equal(bits(dh), bits(dl))
+            assert sl["mlp_rounds"] > 0
+            th.append(sh["t_solve_ms"])
+            tl.append(sl["t_solve_ms"])
+        gain = sum(th) / sum(tl)
+        print(f"learned vs heuristic on RMAT-21: {gain:.3f}x (heuristic {sum(th):.2f} ms, "
+              f"learned {sum(tl):.2f} ms over {len(srcs)} sources)")
+        assert gain > 0.8
+    finally:
+        g.close()
