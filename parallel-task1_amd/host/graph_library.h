@@ -1,0 +1,376 @@
+// graph_library.h -- header-only C++ host side of the MI355X SSSP library.
+//
+// Mirrors the reference library's interface for the single-source shortest
+// path path so that driver code written against it (sample/sssp.cpp,
+// sample/solve.cpp) compiles unchanged against this header:
+//   Graph<V,W> with EDGES_LIST / COMPRESSED_ADJACENCY_LIST containers and
+//     GraphContainerData      (include/graph_data_structures/graph.h:39-210,
+//                              graph_container_data.h:12-26)
+//   GraphGenerationAPI      (include/test/generation/graph_generation_API.h:9-26)
+//   GraphStorageAPI         (include/storage/graph_storage_API.h)
+//   SingleSourceShortestPaths<V,W>::gpu_bellman_ford  (algorithm/sssp/sssp.h:23-41)
+//   init_distances_wrapper<W> / bellman_ford_wrapper<W>
+//                           (algorithm/sssp/gpu/bellman_ford.cuh:13-18)
+// Everything device-side goes through the C ABI of libsssp_mi355x.so
+// (include/sssp_mi355x.h). Failures throw `const char*`, as the reference's
+// SAFE_CALL does (include/common/cuda_error_hadling.h:4-24).
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/sssp_mi355x.h"
+
+namespace sssp_host {
+inline void check(int rc)
+{
+    if (rc != SSSP_OK) {
+        static thread_local std::string msg;
+        msg = std::string("error in HIP library call, aborting...: ") + sssp_last_error();
+        throw msg.c_str();
+    }
+}
+template <typename W> struct WeightType;
+template <> struct WeightType<float> { static const int32_t code = SSSP_W_F32; };
+template <> struct WeightType<unsigned int> { static const int32_t code = SSSP_W_U32; };
+template <> struct WeightType<double> { static const int32_t code = SSSP_W_F64; };
+}  // namespace sssp_host
+
+enum ContainerType { COMPRESSED_ADJACENCY_LIST, ADJACENCY_LIST, EDGES_LIST };
+
+template <typename _TEdgeWeight>
+struct Edge {
+    int src_id;
+    int dst_id;
+    _TEdgeWeight weight;
+    Edge() : src_id(0), dst_id(0), weight(0) {}
+    Edge(int s, int d, _TEdgeWeight w) : src_id(s), dst_id(d), weight(w) {}
+};
+
+template <typename _TVertexValue, typename _TEdgeWeight>
+struct GraphContainerData {
+    int *vertices_ids;
+    int *edges_src_ids;
+    int *edges_dst_ids;
+    long long *vertices_to_edges_ptrs;
+    _TVertexValue *vertices_values;
+    _TEdgeWeight *edges_weights;
+    GraphContainerData()
+        : vertices_ids(NULL), edges_src_ids(NULL), edges_dst_ids(NULL),
+          vertices_to_edges_ptrs(NULL), vertices_values(NULL), edges_weights(NULL) {}
+};
+
+// Graph restricted to the two storage formats the SSSP path uses. EDGES_LIST
+// is SoA (src, dst, weight) in insertion order; COMPRESSED_ADJACENCY_LIST is
+// CSR with long long offsets whose per-vertex order is the edge-list order
+// (what graph.hpp:316-361 produces through its adjacency-list detour).
+template <typename _TVertexValue, typename _TEdgeWeight>
+class Graph {
+  public:
+    Graph(int vertices_count, long long edges_count, ContainerType type, bool directed,
+          bool empty = true)
+        : type_(type), directed_(directed)
+    {
+        resize(vertices_count, edges_count, empty);
+    }
+    Graph() : type_(ADJACENCY_LIST), directed_(true) { resize(0, 0, true); }
+
+    void resize(int vertices_count, long long edges_count, bool empty = true)
+    {
+        if (type_ == ADJACENCY_LIST) type_ = EDGES_LIST;
+        values_.assign((size_t)vertices_count, _TVertexValue());
+        ids_.clear();
+        src_.assign((size_t)edges_count, 0);
+        dst_.assign((size_t)edges_count, 0);
+        w_.assign((size_t)edges_count, _TEdgeWeight());
+        ptrs_.clear();
+        nv_ = empty ? 0 : vertices_count;
+        ne_ = empty ? 0 : edges_count;
+        if (type_ == COMPRESSED_ADJACENCY_LIST) build_ptrs_from_sorted();
+    }
+
+    int get_vertices_count() const { return nv_; }
+    long long get_edges_count() const { return ne_; }
+    bool check_if_directed() const { return directed_; }
+    void set_directed(bool d) { directed_ = d; }
+    ContainerType get_container_type() const { return type_; }
+
+    void add_vertex(int id, _TVertexValue value)
+    {
+        if ((size_t)nv_ >= values_.size()) values_.resize((size_t)nv_ + 1);
+        values_[(size_t)nv_] = value;
+        (void)id;
+        nv_++;
+    }
+
+    void add_edge(int src, int dst, _TEdgeWeight weight)
+    {
+        if (type_ != EDGES_LIST) throw "add_edge not implemented yet";
+        if (!directed_ && src > dst) std::swap(src, dst);  // edges_list.hpp:240-244
+        if ((size_t)ne_ >= src_.size()) {
+            size_t n = std::max<size_t>(1, src_.size() * 2);
+            src_.resize(n);
+            dst_.resize(n);
+            w_.resize(n);
+        }
+        src_[(size_t)ne_] = src;
+        dst_[(size_t)ne_] = dst;
+        w_[(size_t)ne_] = weight;
+        ne_++;
+    }
+
+    Edge<_TEdgeWeight> iterate_edges(long long pos) const
+    {
+        if (type_ != EDGES_LIST) throw "not implemented yet";
+        if (pos >= ne_) throw "Out of bounds in ITERATE_VERTICES function";
+        return Edge<_TEdgeWeight>(src_[(size_t)pos], dst_[(size_t)pos], w_[(size_t)pos]);
+    }
+
+    GraphContainerData<_TVertexValue, _TEdgeWeight> get_graph_data()
+    {
+        GraphContainerData<_TVertexValue, _TEdgeWeight> d;
+        if (type_ == EDGES_LIST) {
+            d.edges_src_ids = src_.data();
+            d.edges_dst_ids = dst_.data();
+            d.edges_weights = w_.data();
+        } else {
+            ids_.resize((size_t)nv_ + 1);
+            for (int i = 0; i <= nv_; i++) ids_[(size_t)i] = i < nv_ ? i : 0;
+            d.vertices_ids = ids_.data();
+            d.vertices_to_edges_ptrs = ptrs_.data();
+            d.vertices_values = values_.data();
+            d.edges_dst_ids = dst_.data();
+            d.edges_weights = w_.data();
+        }
+        return d;
+    }
+
+    void convert_to_edges_list()
+    {
+        if (type_ == EDGES_LIST) return;
+        // CSR rows in vertex order back to an edge list
+        for (int v = 0; v < nv_; v++)
+            for (long long e = ptrs_[(size_t)v]; e < ptrs_[(size_t)v + 1]; e++) src_[(size_t)e] = v;
+        ptrs_.clear();
+        type_ = EDGES_LIST;
+    }
+
+    void convert_to_compressed_adjacency_list()
+    {
+        if (type_ == COMPRESSED_ADJACENCY_LIST) return;
+        // stable counting sort of the edge list by source
+        std::vector<long long> cnt((size_t)nv_ + 1, 0);
+        for (long long e = 0; e < ne_; e++) cnt[(size_t)src_[(size_t)e] + 1]++;
+        for (int v = 0; v < nv_; v++) cnt[(size_t)v + 1] += cnt[(size_t)v];
+        std::vector<int> s2((size_t)ne_), d2((size_t)ne_);
+        std::vector<_TEdgeWeight> w2((size_t)ne_);
+        std::vector<long long> cur(cnt.begin(), cnt.end() - 1);
+        for (long long e = 0; e < ne_; e++) {
+            const long long p = cur[(size_t)src_[(size_t)e]]++;
+            s2[(size_t)p] = src_[(size_t)e];
+            d2[(size_t)p] = dst_[(size_t)e];
+            w2[(size_t)p] = w_[(size_t)e];
+        }
+        src_.swap(s2);
+        dst_.swap(d2);
+        w_.swap(w2);
+        ptrs_.swap(cnt);
+        type_ = COMPRESSED_ADJACENCY_LIST;
+    }
+
+  private:
+    void build_ptrs_from_sorted()
+    {
+        ptrs_.assign((size_t)nv_ + 1, 0);
+        for (long long e = 0; e < ne_; e++) ptrs_[(size_t)src_[(size_t)e] + 1]++;
+        for (int v = 0; v < nv_; v++) ptrs_[(size_t)v + 1] += ptrs_[(size_t)v];
+    }
+
+    ContainerType type_;
+    bool directed_;
+    int nv_ = 0;
+    long long ne_ = 0;
+    std::vector<_TVertexValue> values_;
+    std::vector<int> ids_;
+    std::vector<int> src_, dst_;
+    std::vector<_TEdgeWeight> w_;
+    std::vector<long long> ptrs_;
+};
+
+template <typename _TVertexValue, typename _TEdgeWeight>
+class GraphGenerationAPI {
+  public:
+    // graph_generation_API.hpp:404-484 with time(NULL) replaced by `seed` and
+    // the OpenMP thread count by a fixed virtual thread count (see
+    // sssp_rmat_generate); _omp_threads == 1 reproduces the reference exactly.
+    static void R_MAT_parallel(Graph<_TVertexValue, _TEdgeWeight> &graph, int vertices_count,
+                               int average_connections, int a, int b, int c, int d,
+                               int omp_threads, bool directed, unsigned seed = 1)
+    {
+        if (!directed) throw "only directed R-MAT graphs are supported";
+        if (sssp_host::WeightType<_TEdgeWeight>::code != SSSP_W_F32)
+            throw "R-MAT weights are float (graph_generation_API.hpp:470)";
+        int scale = 0;
+        while ((1 << scale) < vertices_count) scale++;
+        if ((1 << scale) != vertices_count) throw "vertices_count must be a power of two";
+        graph.convert_to_edges_list();
+        graph.resize(vertices_count, (long long)vertices_count * average_connections, false);
+        GraphContainerData<_TVertexValue, _TEdgeWeight> g = graph.get_graph_data();
+        sssp_host::check(sssp_rmat_generate(scale, average_connections, a, b, c, d, seed,
+                                            omp_threads, g.edges_src_ids, g.edges_dst_ids,
+                                            reinterpret_cast<float *>(g.edges_weights)));
+    }
+
+    // graph_generation_API.hpp:4-71 (directed, not optimized) after srand(seed)
+    static void random_uniform(Graph<_TVertexValue, _TEdgeWeight> &graph, int vertices_count,
+                               int average_degree, bool directed, bool optimized, unsigned seed = 1)
+    {
+        if (!directed || optimized) throw "only directed, non-optimized random_uniform is supported";
+        graph.convert_to_edges_list();
+        graph.resize(vertices_count, (long long)vertices_count * average_degree, false);
+        GraphContainerData<_TVertexValue, _TEdgeWeight> g = graph.get_graph_data();
+        const int mode = sssp_host::WeightType<_TEdgeWeight>::code == SSSP_W_F32 ? 0 : 1;
+        sssp_host::check(sssp_er_generate(vertices_count, average_degree, seed, mode,
+                                          g.edges_src_ids, g.edges_dst_ids, g.edges_weights));
+    }
+};
+
+template <typename _TVertexValue, typename _TEdgeWeight>
+class GraphStorageAPI {
+  public:
+    static bool save_to_edges_list_bin_file(Graph<_TVertexValue, _TEdgeWeight> &graph,
+                                            std::string file_name)
+    {
+        graph.convert_to_edges_list();
+        GraphContainerData<_TVertexValue, _TEdgeWeight> g = graph.get_graph_data();
+        return sssp_save_bin(file_name.c_str(), graph.check_if_directed() ? 1 : 0, g.edges_src_ids,
+                             g.edges_dst_ids, g.edges_weights, (int32_t)sizeof(_TEdgeWeight),
+                             graph.get_vertices_count(), graph.get_edges_count()) == SSSP_OK;
+    }
+
+    static bool load_from_edges_list_bin_file(Graph<_TVertexValue, _TEdgeWeight> &graph,
+                                              std::string file_name)
+    {
+        int32_t directed = 0, V = 0;
+        int64_t E = 0;
+        if (sssp_load_bin_header(file_name.c_str(), &directed, &V, &E) != SSSP_OK) return false;
+        if ((directed != 0) != graph.check_if_directed())
+            throw "loading directed graph to non-directed container(or undirected to directed)";
+        graph.convert_to_edges_list();
+        graph.resize(V, E, false);
+        GraphContainerData<_TVertexValue, _TEdgeWeight> g = graph.get_graph_data();
+        sssp_host::check(sssp_load_bin(file_name.c_str(), (int32_t)sizeof(_TEdgeWeight),
+                                       g.edges_src_ids, g.edges_dst_ids, g.edges_weights, E));
+        return true;
+    }
+};
+
+// Reference-shaped device wrappers (bellman_ford.cuh:13-18) over caller-owned
+// device arrays.
+template <typename W> void init_distances_wrapper(W *distances, int vertices_count, int source);
+template <> inline void init_distances_wrapper<float>(float *d, int V, int s)
+{
+    sssp_host::check(sssp_init_distances_f32(d, V, s));
+}
+template <> inline void init_distances_wrapper<double>(double *d, int V, int s)
+{
+    sssp_host::check(sssp_init_distances_f64(d, V, s));
+}
+template <> inline void init_distances_wrapper<unsigned int>(unsigned int *d, int V, int s)
+{
+    sssp_host::check(sssp_init_distances_u32(d, V, s));
+}
+
+template <typename W>
+void bellman_ford_wrapper(int *src_ids, int *dst_ids, W *weights, int vertices_count,
+                          long long edges_count, int source, W *distances);
+template <>
+inline void bellman_ford_wrapper<float>(int *s, int *t, float *w, int V, long long E, int src, float *d)
+{
+    sssp_host::check(sssp_bellman_ford_f32(s, t, w, V, E, src, d, NULL));
+}
+template <>
+inline void bellman_ford_wrapper<double>(int *s, int *t, double *w, int V, long long E, int src,
+                                         double *d)
+{
+    sssp_host::check(sssp_bellman_ford_f64(s, t, w, V, E, src, d, NULL));
+}
+template <>
+inline void bellman_ford_wrapper<unsigned int>(int *s, int *t, unsigned int *w, int V, long long E,
+                                               int src, unsigned int *d)
+{
+    sssp_host::check(sssp_bellman_ford_u32(s, t, w, V, E, src, d, NULL));
+}
+
+// SingleSourceShortestPaths<V,W>::gpu_bellman_ford (sssp.hpp:219-278): the
+// graph is uploaded once and cached (keyed on its data pointers and sizes),
+// the solve runs on the device, the distances are copied to `distances`, and
+// the return value is MTEPS over the solve window (uploads and downloads
+// excluded, as the reference's t2..t3).
+template <typename _TVertexValue, typename _TEdgeWeight>
+class SingleSourceShortestPaths {
+  public:
+    explicit SingleSourceShortestPaths(int omp_threads = 1, int device = 0,
+                                       int schedule = SSSP_SCHED_HEURISTIC)
+        : omp_threads_(omp_threads), device_(device), schedule_(schedule) {}
+    ~SingleSourceShortestPaths()
+    {
+        if (g_) sssp_graph_destroy(g_);
+        if (mlp_) sssp_mlp_destroy(mlp_);
+    }
+    SingleSourceShortestPaths(const SingleSourceShortestPaths &) = delete;
+    SingleSourceShortestPaths &operator=(const SingleSourceShortestPaths &) = delete;
+
+    void set_omp_threads(int n) { omp_threads_ = n; }
+    void set_schedule(int schedule) { schedule_ = schedule; }
+
+    // Loads scheduler weights (SSSP_MLP_NPARAM floats) for SSSP_SCHED_LEARNED.
+    void set_scheduler_weights(const float *params)
+    {
+        if (!mlp_) sssp_host::check(sssp_mlp_create(device_, 0, &mlp_));
+        sssp_host::check(sssp_mlp_set_params(mlp_, params));
+        if (g_) sssp_host::check(sssp_graph_set_mlp(g_, mlp_));
+    }
+
+    double gpu_bellman_ford(Graph<_TVertexValue, _TEdgeWeight> &graph, _TEdgeWeight *distances,
+                            int source_vertex)
+    {
+        graph.convert_to_edges_list();
+        GraphContainerData<_TVertexValue, _TEdgeWeight> d = graph.get_graph_data();
+        const int V = graph.get_vertices_count();
+        const long long E = graph.get_edges_count();
+        if (!g_ || key_src_ != d.edges_src_ids || key_dst_ != d.edges_dst_ids ||
+            key_w_ != (const void *)d.edges_weights || key_v_ != V || key_e_ != E) {
+            if (g_) sssp_graph_destroy(g_);
+            g_ = NULL;
+            sssp_host::check(sssp_graph_create(sssp_host::WeightType<_TEdgeWeight>::code,
+                                               d.edges_src_ids, d.edges_dst_ids, d.edges_weights, V,
+                                               E, device_, 0, &g_));
+            if (mlp_) sssp_host::check(sssp_graph_set_mlp(g_, mlp_));
+            key_src_ = d.edges_src_ids;
+            key_dst_ = d.edges_dst_ids;
+            key_w_ = d.edges_weights;
+            key_v_ = V;
+            key_e_ = E;
+        }
+        sssp_stats st;
+        sssp_host::check(sssp_solve(g_, source_vertex, schedule_, distances, &st));
+        return E / (st.t_solve_ms * 1e-3 * 1e6);
+    }
+
+  private:
+    int omp_threads_;
+    int device_;
+    int schedule_;
+    sssp_graph *g_ = NULL;
+    sssp_mlp *mlp_ = NULL;
+    const int *key_src_ = NULL;
+    const int *key_dst_ = NULL;
+    const void *key_w_ = NULL;
+    int key_v_ = 0;
+    long long key_e_ = 0;
+};

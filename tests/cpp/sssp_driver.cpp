@@ -1,0 +1,101 @@
+// Driver in the shape of the reference's sample/sssp.cpp / sample/solve.cpp,
+// compiled against parallel-task1_amd/host/graph_library.h:
+//   sssp_driver cpu <dir>        containers, generator and storage only
+//   sssp_driver gpu <dir>        plus gpu_bellman_ford and the reference-shaped
+//                                device wrappers; writes distances to <dir>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "graph_library.h"
+
+static void write_f32(const std::string &path, const float *d, int n)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) throw "cannot write output";
+    std::fwrite(d, sizeof(float), (size_t)n, f);
+    std::fclose(f);
+}
+
+int main(int argc, char **argv)
+{
+    try {
+        if (argc < 3) {
+            std::printf("usage: %s cpu|gpu <dir>\n", argv[0]);
+            return 2;
+        }
+        const bool gpu = std::strcmp(argv[1], "gpu") == 0;
+        const std::string dir = argv[2];
+        const int scale = 12, ef = 16, V = 1 << scale;
+
+        // generate, save, load (sample/graph_generator.cpp -> sample/sssp.cpp:93)
+        Graph<int, float> gen(0, 0, EDGES_LIST, true);
+        GraphGenerationAPI<int, float>::R_MAT_parallel(gen, V, ef, 57, 19, 19, 5, 64, true, 11);
+        const std::string file = dir + "/rmat12.bin";
+        if (!GraphStorageAPI<int, float>::save_to_edges_list_bin_file(gen, file)) throw "save failed";
+        Graph<int, float> graph(0, 0, EDGES_LIST, true);
+        if (!GraphStorageAPI<int, float>::load_from_edges_list_bin_file(graph, file)) throw "load failed";
+        if (graph.get_vertices_count() != V || graph.get_edges_count() != (long long)V * ef)
+            throw "loaded sizes differ";
+        for (long long e = 0; e < graph.get_edges_count(); e += 997) {
+            Edge<float> a = gen.iterate_edges(e), b = graph.iterate_edges(e);
+            if (a.src_id != b.src_id || a.dst_id != b.dst_id || a.weight != b.weight)
+                throw "round trip differs";
+        }
+        // CSR conversion keeps per-vertex edge-list order and converts back
+        Graph<int, float> csr(0, 0, EDGES_LIST, true);
+        GraphStorageAPI<int, float>::load_from_edges_list_bin_file(csr, file);
+        csr.convert_to_compressed_adjacency_list();
+        GraphContainerData<int, float> cd = csr.get_graph_data();
+        long long prev = 0;
+        for (int v = 0; v < V; v++) {
+            if (cd.vertices_to_edges_ptrs[v] < prev) throw "CSR offsets decrease";
+            prev = cd.vertices_to_edges_ptrs[v];
+        }
+        if (cd.vertices_to_edges_ptrs[V] != (long long)V * ef) throw "CSR total differs";
+        csr.convert_to_edges_list();
+        std::printf("containers, generator and storage ok\n");
+        if (!gpu) return 0;
+
+        // the reference driver's loop, GPU path (sample/sssp.cpp:110-151)
+        SingleSourceShortestPaths<int, float> operation;
+        operation.set_omp_threads(1);
+        std::vector<float> result((size_t)V);
+        double mteps = 0;
+        const int sources[3] = {0, 5, 77};
+        for (int i = 0; i < 3; i++) {
+            mteps += operation.gpu_bellman_ford(graph, result.data(), sources[i]);
+            write_f32(dir + "/dist_" + std::to_string(sources[i]) + ".f32", result.data(), V);
+        }
+        std::printf("avg perfomance: %.1f MTEPS\n", mteps / 3);
+
+        // reference-shaped wrappers on device arrays owned by the caller
+        GraphContainerData<int, float> d = graph.get_graph_data();
+        const long long E = graph.get_edges_count();
+        int *ds = NULL, *dd = NULL;
+        float *dw = NULL, *dist = NULL;
+        if (hipMalloc((void **)&ds, E * 4) || hipMalloc((void **)&dd, E * 4) ||
+            hipMalloc((void **)&dw, E * 4) || hipMalloc((void **)&dist, V * 4))
+            throw "hipMalloc failed";
+        if (hipMemcpy(ds, d.edges_src_ids, E * 4, hipMemcpyHostToDevice) ||
+            hipMemcpy(dd, d.edges_dst_ids, E * 4, hipMemcpyHostToDevice) ||
+            hipMemcpy(dw, d.edges_weights, E * 4, hipMemcpyHostToDevice))
+            throw "hipMemcpy failed";
+        init_distances_wrapper<float>(dist, V, 5);
+        bellman_ford_wrapper<float>(ds, dd, dw, V, E, 5, dist);
+        if (hipMemcpy(result.data(), dist, V * 4, hipMemcpyDeviceToHost)) throw "hipMemcpy failed";
+        write_f32(dir + "/wrap_5.f32", result.data(), V);
+        (void)hipFree(ds);
+        (void)hipFree(dd);
+        (void)hipFree(dw);
+        (void)hipFree(dist);
+        std::printf("gpu path ok\n");
+    } catch (const char *error) {
+        std::printf("%s\n", error);
+        return 1;
+    }
+    return 0;
+}
