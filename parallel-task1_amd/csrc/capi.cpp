@@ -42,7 +42,8 @@ void graph_free(sssp_graph *g)
                     g->nzo_v, g->nzo_scan, g->nzi_v, g->nzi_scan, g->sample_v, g->dist, g->stamp,
                     g->queue[0], g->queue[1], g->lb_v, g->lb_start, g->lb_scan, g->lb_tile,
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
-                    g->src_list, g->btasks, g->bcounter, g->bround};
+                    g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
+                    g->x_w, g->xgoff};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {
@@ -374,7 +375,10 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
             g->max_rounds = (int)value;
         } else if (n == "push_edge_frac") g->push_edge_frac = value;
         else if (n == "pull_frac") g->pull_frac = value;
-        else if (n == "trace_events") g->trace_events = value != 0;
+        else if (n == "delta") {
+            SSSP_REQUIRE(value >= 0, "delta must be >= 0");
+            g->delta = value;
+        } else if (n == "trace_events") g->trace_events = value != 0;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};
     });
 }
@@ -392,6 +396,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "max_rounds") *value = g->max_rounds;
         else if (n == "push_edge_frac") *value = g->push_edge_frac;
         else if (n == "pull_frac") *value = g->pull_frac;
+        else if (n == "delta") *value = g->delta;
         else if (n == "trace_events") *value = g->trace_events;
         else if (n == "cu_count") *value = g->cu_count;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};

@@ -24,6 +24,8 @@
 //   converge:  stop after a pass that changes nothing          (:126-141)
 // The plain schedule (SSSP_SCHED_PLAIN) is exactly that loop over the
 // src-sorted edge list, with the convergence flag tested on the device.
+#include <cstring>
+
 #include "mlp_dev.hpp"
 #include "sssp_graph.hpp"
 
@@ -58,6 +60,8 @@ struct AdvArgs {
     int32_t *qnext;
     RoundSlot *slot;
     uint32_t minF;
+    const int64_t *dev_n;      // when set, entry count and edge total are read
+    const int64_t *dev_T;      // from the device (near-far compaction)
 };
 
 constexpr int kStageCap = 512;  // per-wave LDS staging slots for frontier appends
@@ -141,6 +145,11 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
     int32_t *wbuf = s_stage[tid >> 6];
     int wcnt = 0;
     RoundAcc acc;
+    if (a.dev_n) {
+        a.n = (int32_t)*a.dev_n;
+        a.T = *a.dev_T;
+        a.ntiles = (a.T + TE - 1) / TE;
+    }
 
     // XCD-aware tile order: groups of 16 consecutive tiles stay on one XCD
     // (blocks are dispatched round-robin over the 8 XCDs); groups interleave
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
                                                       uint32_t *dist, int32_t *stamp,
                                                       int32_t tick_prev, int32_t tick_cur,
                                                       const int64_t *row_ptr, int32_t *qnext,
-                                                      RoundSlot *slot, int64_t per_block)
+                                                      RoundSlot *slot, const int64_t *goff)
 {
     typedef int32_t v4i __attribute__((ext_vector_type(4)));
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
@@ -276,28 +285,27 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
     int32_t *wbuf = s_stage[threadIdx.x >> 6];
     int wcnt = 0;
     RoundAcc acc;
-    const int64_t ng = E >> 2;
-    const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t g0 = b * per_block, g1 = min(g0 + per_block, ng);
+    // XCD-partitioned list (see k_el_pass): group blockIdx % 8, padded to
+    // whole 16-byte vectors
+    const int64_t x = blockIdx.x & 7, nbx = gridDim.x >> 3, bi = blockIdx.x >> 3;
+    const int64_t ga = goff[x] >> 2, gb = goff[x + 1] >> 2;
+    const int64_t per = (gb - ga + nbx - 1) / nbx;
+    const int64_t g0 = ga + bi * per, g1 = min(g0 + per, gb);
     const int64_t iters = g1 > g0 ? (g1 - g0 + kBlock - 1) / kBlock : 0;
     const v4i *s4 = reinterpret_cast<const v4i *>(src);
     const v4i *t4 = reinterpret_cast<const v4i *>(dst);
     const v4u *w4 = reinterpret_cast<const v4u *>(w);
-    for (int64_t it = 0; it <= iters; it++) {
-        // the last iteration handles the E % 4 tail edges in block 0
+    for (int64_t it = 0; it < iters; it++) {
         int32_t es[4], et[4];
         uint32_t ew[4];
         bool val[4] = {false, false, false, false};
-        if (it < iters) {
+        {
             const int64_t g = g0 + it * kBlock + threadIdx.x;
             if (g < g1) {
                 const v4i sv = __builtin_nontemporal_load(&s4[g]);
                 es[0] = sv.x; es[1] = sv.y; es[2] = sv.z; es[3] = sv.w;
                 val[0] = val[1] = val[2] = val[3] = true;
             }
-        } else if (b == 0 && threadIdx.x < (E & 3)) {
-            es[0] = src[(ng << 2) + threadIdx.x];
-            val[0] = true;
         }
         uint32_t du[4];
         bool any = false;
@@ -312,17 +320,15 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
                 any |= act;
             }
         }
+        uint32_t dv[4];
         if (any) {
-            if (it < iters) {
-                const int64_t g = g0 + it * kBlock + threadIdx.x;
-                const v4i tv = __builtin_nontemporal_load(&t4[g]);
-                const v4u wv = __builtin_nontemporal_load(&w4[g]);
-                et[0] = tv.x; et[1] = tv.y; et[2] = tv.z; et[3] = tv.w;
-                ew[0] = wv.x; ew[1] = wv.y; ew[2] = wv.z; ew[3] = wv.w;
-            } else {
-                et[0] = dst[(ng << 2) + threadIdx.x];
-                ew[0] = w[(ng << 2) + threadIdx.x];
-            }
+            const int64_t g = g0 + it * kBlock + threadIdx.x;
+            const v4i tv = __builtin_nontemporal_load(&t4[g]);
+            const v4u wv = __builtin_nontemporal_load(&w4[g]);
+            et[0] = tv.x; et[1] = tv.y; et[2] = tv.z; et[3] = tv.w;
+            ew[0] = wv.x; ew[1] = wv.y; ew[2] = wv.z; ew[3] = wv.w;
+#pragma unroll
+            for (int i = 0; i < 4; i++) dv[i] = du[i] != INF ? dist[et[i]] : 0u;
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -332,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
                 v = et[i];
                 const uint32_t cand = WT::add(du[i], ew[i]);
                 acc.relaxed++;
-                if (cand < dist[v]) {
+                if (cand < dv[i]) {
                     const uint32_t old = atomicMin(&dist[v], cand);
                     if (cand < old) {
                         low = true;
@@ -351,66 +357,135 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Sparse-frontier preparation: degree sums per block, block scan, entries.
+// Sparse-frontier preparation with a near-far split: queue entries whose
+// distance is below `thr` ("near") become load-balancing entries (vertex, row
+// start, exclusive degree scan, tile table); the others ("far") are carried
+// unprocessed into the next frontier, so every vertex with unrelaxed
+// out-edges stays queued. thr = 0xFFFFFFFF disables the split.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_lb_reduce(const int32_t *q, int32_t n,
-                                                      const int64_t *row_ptr, int64_t *blk)
+                                                      const int64_t *row_ptr, const uint32_t *dist,
+                                                      uint32_t thr, int64_t *blk_deg,
+                                                      int64_t *blk_cnt)
 {
     __shared__ int64_t s_warp[kBlock / 64];
     const int64_t base = (int64_t)blockIdx.x * kLbTile + (int64_t)threadIdx.x * kLbItems;
-    int64_t s = 0;
+    int64_t s = 0, c = 0;
 #pragma unroll
     for (int i = 0; i < kLbItems; i++) {
         const int64_t j = base + i;
         if (j < n) {
             const int32_t v = q[j];
-            s += row_ptr[v + 1] - row_ptr[v];
+            if (dist[v] < thr) {
+                s += row_ptr[v + 1] - row_ptr[v];
+                c++;
+            }
         }
     }
-    int64_t tot;
-    (void)block_excl_scan_i64(s, &tot, s_warp);
-    if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+    int64_t tot_s, tot_c;
+    (void)block_excl_scan_i64(s, &tot_s, s_warp);
+    (void)block_excl_scan_i64(c, &tot_c, s_warp);
+    if (threadIdx.x == 0) {
+        blk_deg[blockIdx.x] = tot_s;
+        blk_cnt[blockIdx.x] = tot_c;
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_lb_write(const int32_t *q, int32_t n,
-                                                     const int64_t *row_ptr, const int64_t *blk,
-                                                     int32_t *lv, int64_t *ls, int64_t *lscan,
-                                                     int32_t *ltile, int64_t TE, int64_t ntiles)
+struct LbWriteArgs {
+    const int32_t *q;
+    int32_t n;
+    const int64_t *row_ptr;
+    const uint32_t *dist;
+    uint32_t thr;
+    const int64_t *blk_deg;  // scanned, [nb] = total edges
+    const int64_t *blk_cnt;  // scanned, [nb] = near entries
+    int64_t nb;
+    int32_t *lv;
+    int64_t *ls;
+    int64_t *lscan;
+    int32_t *ltile;
+    int64_t TE;
+    Ctrl *ctrl;              // receives n_compact / lb_total
+    int32_t *stamp;          // far carry: dedup against this round's appends
+    int32_t tick_cur;
+    int32_t *qnext;
+    RoundSlot *slot;
+};
+
+__global__ __launch_bounds__(kBlock) void k_lb_write(LbWriteArgs a)
 {
     __shared__ int64_t s_warp[kBlock / 64];
     const int64_t base = (int64_t)blockIdx.x * kLbTile + (int64_t)threadIdx.x * kLbItems;
     int32_t vv[kLbItems];
+    uint32_t dd[kLbItems];
     int64_t st[kLbItems], dg[kLbItems];
-    int64_t s = 0;
+    bool nr[kLbItems];
+    int64_t s = 0, c = 0;
 #pragma unroll
     for (int i = 0; i < kLbItems; i++) {
         const int64_t j = base + i;
         vv[i] = 0;
+        dd[i] = 0xFFFFFFFFu;
         st[i] = 0;
         dg[i] = 0;
-        if (j < n) {
-            vv[i] = q[j];
-            st[i] = row_ptr[vv[i]];
-            dg[i] = row_ptr[vv[i] + 1] - st[i];
+        nr[i] = false;
+        if (j < a.n) {
+            vv[i] = a.q[j];
+            dd[i] = a.dist[vv[i]];
+            st[i] = a.row_ptr[vv[i]];
+            dg[i] = a.row_ptr[vv[i] + 1] - st[i];
+            nr[i] = dd[i] < a.thr;
         }
-        s += dg[i];
+        if (nr[i]) {
+            s += dg[i];
+            c++;
+        }
     }
+    const int64_t total_edges = a.blk_deg[a.nb], total_near = a.blk_cnt[a.nb];
     int64_t tot;
-    int64_t off = block_excl_scan_i64(s, &tot, s_warp) + blk[blockIdx.x];
+    int64_t off = block_excl_scan_i64(s, &tot, s_warp) + a.blk_deg[blockIdx.x];
+    int64_t idx = block_excl_scan_i64(c, &tot, s_warp) + a.blk_cnt[blockIdx.x];
+    const int64_t ntiles = (total_edges + a.TE - 1) / a.TE;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctrl->n_compact = total_near;
+        a.ctrl->lb_total = total_edges;
+        if (total_near == 0) a.ltile[0] = 0;
+    }
+    int64_t carried_edges = 0;
+    uint32_t carried_min = 0xFFFFFFFFu;
 #pragma unroll
     for (int i = 0; i < kLbItems; i++) {
-        const int64_t j = base + i;
-        if (j < n) {
-            lv[j] = vv[i];
-            ls[j] = st[i];
-            lscan[j] = off;
-            for (int64_t t = (off + TE - 1) / TE; t * TE < off + dg[i]; t++) ltile[t] = (int32_t)j;
+        if (nr[i]) {
+            const int64_t j = idx;
+            a.lv[j] = vv[i];
+            a.ls[j] = st[i];
+            a.lscan[j] = off;
+            for (int64_t t = (off + a.TE - 1) / a.TE; t * a.TE < off + dg[i]; t++)
+                a.ltile[t] = (int32_t)j;
             off += dg[i];
-            if (j == n - 1) {
-                lscan[n] = off;
-                ltile[ntiles] = n;
+            idx++;
+            if (j == total_near - 1) {
+                a.lscan[total_near] = off;
+                a.ltile[ntiles] = (int32_t)total_near;
             }
         }
+        // far entries go on to the next frontier (once per round)
+        bool carry = false;
+        if (base + i < a.n && !nr[i]) {
+            carry = atomicExch(&a.stamp[vv[i]], a.tick_cur) != a.tick_cur;
+            if (carry) {
+                carried_edges += dg[i];
+                carried_min = min(carried_min, dd[i]);
+            }
+        }
+        wave_append(carry, vv[i], a.qnext, &a.slot->qlen);
+    }
+    carried_edges = wave_sum64(carried_edges);
+    carried_min = wave_min_u32(carried_min);
+    if (__lane_id() == 0) {
+        if (carried_edges)
+            atomicAdd((unsigned long long *)&a.slot->qedges, (unsigned long long)carried_edges);
+        if (carried_min != 0xFFFFFFFFu) atomicMin(&a.slot->minw, carried_min);
     }
 }
 
@@ -602,31 +677,44 @@ __global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
 // the flag pass p+1 will set, and returns at once when pass p-1 changed
 // nothing, so several passes can be queued without a host round trip.
 // ---------------------------------------------------------------------------
-template <class WT, class DT, int VEC>
+template <class WT, class DT, int VEC, bool GROUPED>
 __global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ src,
                                                     const int32_t *__restrict__ dst,
                                                     const DT *__restrict__ w, int64_t E,
                                                     DT *dist, int32_t *changed, int32_t pass,
-                                                    int64_t per_block)
+                                                    int64_t per_block, const int64_t *goff)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) changed[(pass + 1) & 63] = 0;
     if (pass > 0 && __hip_atomic_load(&changed[(pass - 1) & 63], __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT) == 0)
         return;
-    const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
     bool low = false;
-    auto relax = [&](int32_t s, int32_t t, DT ww) {
-        const DT du = dist[s];
+    // both distances of an edge are loaded before either is used, so the
+    // random d[dst] gather does not wait for the d[src] test
+    auto relax = [&](DT du, DT dv, int32_t t, DT ww) {
         if (du == WT::INF) return;
         const DT cand = WT::add(du, ww);
-        if (cand < dist[t]) {
+        if (cand < dv) {
             const DT old = atomicMin(&dist[t], cand);
             low |= cand < old;
         }
     };
     if constexpr (VEC == 4) {
-        const int64_t ng = E >> 2;
-        const int64_t g0 = b * per_block, g1 = min(g0 + per_block, ng);
+        int64_t g0, g1;
+        if constexpr (GROUPED) {
+            // XCD-partitioned list: the workgroups of XCD x (blockIdx % 8 == x,
+            // round-robin dispatch) share group x, whose destinations map to
+            // a disjoint eighth of the distance array
+            const int64_t x = blockIdx.x & 7, nbx = gridDim.x >> 3, bi = blockIdx.x >> 3;
+            const int64_t ga = goff[x] >> 2, gb = goff[x + 1] >> 2;
+            const int64_t per = (gb - ga + nbx - 1) / nbx;
+            g0 = ga + bi * per;
+            g1 = min(g0 + per, gb);
+        } else {
+            const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+            g0 = b * per_block;
+            g1 = min(g0 + per_block, E >> 2);
+        }
         typedef int32_t v4i __attribute__((ext_vector_type(4)));
         typedef uint32_t v4u __attribute__((ext_vector_type(4)));
         const v4i *s4 = reinterpret_cast<const v4i *>(src);
@@ -636,18 +724,24 @@ __global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ 
             const v4i s = __builtin_nontemporal_load(&s4[g]);
             const v4i t = __builtin_nontemporal_load(&t4[g]);
             const v4u ww = __builtin_nontemporal_load(&w4[g]);
-            relax(s.x, t.x, (DT)ww.x);
-            relax(s.y, t.y, (DT)ww.y);
-            relax(s.z, t.z, (DT)ww.z);
-            relax(s.w, t.w, (DT)ww.w);
+            const DT u0 = dist[s.x], u1 = dist[s.y], u2 = dist[s.z], u3 = dist[s.w];
+            const DT v0 = dist[t.x], v1 = dist[t.y], v2 = dist[t.z], v3 = dist[t.w];
+            relax(u0, v0, t.x, (DT)ww.x);
+            relax(u1, v1, t.y, (DT)ww.y);
+            relax(u2, v2, t.z, (DT)ww.z);
+            relax(u3, v3, t.w, (DT)ww.w);
         }
-        if (b == 0 && threadIdx.x < (E & 3)) {
-            const int64_t e = (ng << 2) + threadIdx.x;
-            relax(src[e], dst[e], w[e]);
+        if constexpr (!GROUPED) {
+            if (xcd_tile(blockIdx.x, gridDim.x) == 0 && threadIdx.x < (E & 3)) {
+                const int64_t e = ((E >> 2) << 2) + threadIdx.x;
+                relax(dist[src[e]], dist[dst[e]], dst[e], w[e]);
+            }
         }
     } else {
+        const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
         const int64_t e0 = b * per_block, e1 = min(e0 + per_block, E);
-        for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock) relax(src[e], dst[e], w[e]);
+        for (int64_t e = e0 + threadIdx.x; e < e1; e += kBlock)
+            relax(dist[src[e]], dist[dst[e]], dst[e], w[e]);
     }
     if (__ballot(low) != 0 && __lane_id() == 0) changed[pass & 63] = 1;
 }
@@ -764,19 +858,42 @@ void launch_advance(sssp_graph *g, Kind kind, int kidx, AdvArgs &a)
     }
 }
 
-// Builds the entries of a sparse frontier (queue with out-degree > 0 only).
-void lb_prepare(sssp_graph *g, const int32_t *q, int32_t n, int64_t total, int kidx)
+// Near-far threshold of a push round: entries with d < min over the frontier
+// + delta are relaxed, the rest wait. Always above the minimum, so every push
+// round relaxes at least the frontier's closest vertex.
+uint32_t near_threshold(int wtype, uint32_t minF, double delta)
+{
+    if (!(delta > 0) || minF == 0xFFFFFFFFu) return 0xFFFFFFFFu;
+    uint32_t thr;
+    if (wtype == SSSP_W_F32) {
+        float f;
+        std::memcpy(&f, &minF, 4);
+        f = f + (float)delta;
+        std::memcpy(&thr, &f, 4);
+        if (!(f < 3.0e38f)) thr = 0xFFFFFFFFu;
+    } else {
+        const uint64_t t = (uint64_t)minF + (uint64_t)(delta < 4e9 ? delta : 4e9);
+        thr = t >= 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)t;
+    }
+    return thr > minF ? thr : minF + 1;
+}
+
+// Builds the near entries of a sparse frontier (queue with out-degree > 0
+// only) and carries the far ones into the next queue.
+void lb_prepare(sssp_graph *g, const int32_t *q, int32_t n, int kidx, uint32_t thr,
+                int32_t tick_cur, int32_t *qnext, RoundSlot *slot_next)
 {
     hipStream_t s = g->stream;
     const int64_t nb = (n + kLbTile - 1) / kLbTile;
     if (nb + 1 > g->n_blk) throw Error{SSSP_ERR_INTERNAL, "lb scratch too small"};
     const int64_t TE = (int64_t)kBlock * kChunkK[kidx];
-    const int64_t ntiles = (total + TE - 1) / TE;
-    k_lb_reduce<<<(unsigned)nb, kBlock, 0, s>>>(q, n, g->row_ptr, g->blk_a);
+    k_lb_reduce<<<(unsigned)nb, kBlock, 0, s>>>(q, n, g->row_ptr, g->dist, thr, g->blk_a, g->blk_b);
     SSSP_LAUNCH_CHECK("k_lb_reduce");
     scan_blocks_inplace(g->blk_a, nb, s);
-    k_lb_write<<<(unsigned)nb, kBlock, 0, s>>>(q, n, g->row_ptr, g->blk_a, g->lb_v, g->lb_start,
-                                                g->lb_scan, g->lb_tile, TE, ntiles);
+    scan_blocks_inplace(g->blk_b, nb, s);
+    LbWriteArgs w{q, n, g->row_ptr, g->dist, thr, g->blk_a, g->blk_b, nb, g->lb_v, g->lb_start,
+                  g->lb_scan, g->lb_tile, TE, g->ctrl, g->stamp, tick_cur, qnext, slot_next};
+    k_lb_write<<<(unsigned)nb, kBlock, 0, s>>>(w);
     SSSP_LAUNCH_CHECK("k_lb_write");
 }
 
@@ -805,30 +922,31 @@ void run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
         // all-edge round streamed over the src-sorted edge list; the chunk
         // bucket sets the grid (2, 4, 8 or 16 workgroups per CU)
         hipStream_t s = g->stream;
-        const int64_t ng = g->E >> 2;
         const int64_t nblocks = (int64_t)g->cu_count * (2 << kidx);
-        const int64_t per_block = (ng + nblocks - 1) / nblocks;
         if (verify)
             k_el_active<WT, PRED_REACHED><<<(unsigned)nblocks, kBlock, 0, s>>>(
-                g->el_src, g->el_dst, g->el_w, g->E, g->dist, g->stamp, a.tick_prev, a.tick_cur,
-                g->row_ptr, a.qnext, a.slot, per_block);
+                g->x_src, g->x_dst, g->x_w, g->E, g->dist, g->stamp, a.tick_prev, a.tick_cur,
+                g->row_ptr, a.qnext, a.slot, g->xgoff);
         else
             k_el_active<WT, PRED_STAMP><<<(unsigned)nblocks, kBlock, 0, s>>>(
-                g->el_src, g->el_dst, g->el_w, g->E, g->dist, g->stamp, a.tick_prev, a.tick_cur,
-                g->row_ptr, a.qnext, a.slot, per_block);
+                g->x_src, g->x_dst, g->x_w, g->E, g->dist, g->stamp, a.tick_prev, a.tick_cur,
+                g->row_ptr, a.qnext, a.slot, g->xgoff);
         SSSP_LAUNCH_CHECK("k_el_active");
         return;
     }
     switch (mode) {
     case SSSP_MODE_PUSH: {
         const int32_t *q = g->queue[r & 1];
-        lb_prepare(g, q, cur.qlen, cur.qedges, kidx);
+        const uint32_t thr = near_threshold(g->wtype, cur.minw, g->delta);
+        lb_prepare(g, q, cur.qlen, kidx, thr, a.tick_cur, a.qnext, a.slot);
         a.ent_v = g->lb_v;
         a.ent_start = g->lb_start;
         a.ent_scan = g->lb_scan;
         a.tile = g->lb_tile;
-        a.n = cur.qlen;
-        a.T = cur.qedges;
+        a.n = cur.qlen;     // upper bounds for the grid; the kernel reads the
+        a.T = cur.qedges;   // compacted counts from the device
+        a.dev_n = &g->ctrl->n_compact;
+        a.dev_T = &g->ctrl->lb_total;
         a.adj = g->el_dst;
         a.wts = g->el_w;
         launch_advance<WT>(g, Kind::SparsePush, kidx, a);
@@ -878,8 +996,9 @@ void solve_plain(sssp_graph *g, sssp_stats *st)
     while (done_at < 0) {
         for (int i = 0; i < batch; i++) {
             if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
-            k_el_pass<WT, uint32_t, 4><<<(unsigned)nblocks, kBlock, 0, s>>>(
-                g->el_src, g->el_dst, g->el_w, E, g->dist, g->ctrl->changed, pass + i, per_block);
+            k_el_pass<WT, uint32_t, 4, true><<<(unsigned)nblocks, kBlock, 0, s>>>(
+                g->x_src, g->x_dst, g->x_w, E, g->dist, g->ctrl->changed, pass + i, per_block,
+                g->xgoff);
             SSSP_LAUNCH_CHECK("k_el_pass");
             if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r1, s));
         }
@@ -1076,8 +1195,8 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
         read_slot(g, r);
         const RoundSlot cur = g->ctrl_host->slot[r & 1];
         if (cur.qlen == 0) break;
-        lb_prepare(g, g->queue[r & 1], cur.qlen, cur.qedges, 2);
         AdvArgs a = base_args<WT>(g, r, 0);
+        lb_prepare(g, g->queue[r & 1], cur.qlen, 2, 0xFFFFFFFFu, a.tick_cur, a.qnext, a.slot);
         a.ent_v = g->lb_v;
         a.ent_start = g->lb_start;
         a.ent_scan = g->lb_scan;
@@ -1247,11 +1366,11 @@ void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int6
     while (done_at < 0) {
         for (int i = 0; i < 4; i++) {
             if (vec)
-                k_el_pass<WT, DT, 4><<<(unsigned)nblocks, kBlock>>>(src, dst, w, E, d, changed,
-                                                                    pass + i, per_block);
+                k_el_pass<WT, DT, 4, false><<<(unsigned)nblocks, kBlock>>>(
+                    src, dst, w, E, d, changed, pass + i, per_block, nullptr);
             else
-                k_el_pass<WT, DT, 1><<<(unsigned)nblocks, kBlock>>>(src, dst, w, E, d, changed,
-                                                                    pass + i, per_block);
+                k_el_pass<WT, DT, 1, false><<<(unsigned)nblocks, kBlock>>>(
+                    src, dst, w, E, d, changed, pass + i, per_block, nullptr);
             SSSP_LAUNCH_CHECK("bellman_ford pass");
         }
         SSSP_HIP(hipMemcpy(hc, changed, sizeof(hc), hipMemcpyDeviceToHost));

@@ -199,6 +199,60 @@ __global__ __launch_bounds__(kBlock) void k_copy_i32(const uint32_t *in, int32_t
         out[i] = (int32_t)in[i];
 }
 
+// XCD group of an edge: a multiplicative hash of its destination's 64-vertex
+// block, so each XCD's L2 serves a disjoint, work-balanced eighth of the
+// distance array (R-MAT's per-bit skew makes any bit-sliced split uneven).
+__device__ __forceinline__ uint32_t xcd_group(int32_t dst)
+{
+    return (((uint32_t)dst >> 6) * 0x9E3779B1u) >> 29;
+}
+
+__global__ __launch_bounds__(kBlock) void k_group_keys(const int32_t *dst, int64_t E, uint32_t *keys,
+                                                       int64_t *gcount)
+{
+    __shared__ int64_t h[8];
+    if (threadIdx.x < 8) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * kBlock) {
+        const uint32_t k = xcd_group(dst[e]);
+        keys[e] = k;
+        atomicAdd((unsigned long long *)&h[k], 1ull);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && h[threadIdx.x])
+        atomicAdd((unsigned long long *)&gcount[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// out[goff[x] + j] = in[perm[cbase[x] + j]] for j < count[x]; the rest of
+// each group's region (up to 3 slots, so every group is a whole number of
+// 16-byte vectors) is a weight-0 self loop on vertex 0, which never changes
+// a distance.
+__global__ __launch_bounds__(kBlock) void k_group_gather(const uint32_t *perm, const int64_t *goff,
+                                                         const int64_t *cbase, const int32_t *src,
+                                                         const int32_t *dst, const uint32_t *w,
+                                                         int32_t *xs, int32_t *xd, uint32_t *xw)
+{
+    const int64_t Et = goff[8];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < Et;
+         i += (int64_t)gridDim.x * kBlock) {
+        int x = 0;
+        while (x < 7 && i >= goff[x + 1]) x++;
+        const int64_t j = i - goff[x];
+        const int64_t cnt = cbase[x + 1] - cbase[x];
+        if (j < cnt) {
+            const uint32_t p = perm[cbase[x] + j];
+            xs[i] = src[p];
+            xd[i] = dst[p];
+            xw[i] = w[p];
+        } else {
+            xs[i] = 0;
+            xd[i] = 0;
+            xw[i] = 0u;
+        }
+    }
+}
+
 // row_ptr[v] = first position whose sorted key is >= v; row_ptr[V] = E.
 __global__ __launch_bounds__(kBlock) void k_rowptr_from_sorted(const uint32_t *keys, int64_t E,
                                                                int32_t V, int64_t *row_ptr)
@@ -409,6 +463,55 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
     SSSP_LAUNCH_CHECK("k_rowptr_from_sorted");
     build_static_list(g->row_ptr, V, E, counts, tmp_pos, g->blk_a, g->blk_b, g->n_blk, s,
                       &g->nzo_v, &g->nzo_scan, &g->n_nzo, g->nzo_tile);
+
+    // ---- XCD-partitioned edge list: stable split of the src-sorted list into
+    // 8 destination groups (src order kept inside each group) ----
+    {
+        int64_t *gcnt = dmalloc<int64_t>(8);
+        struct F {
+            void *p;
+            ~F() { dfree(p); }
+        } fg{gcnt};
+        SSSP_HIP(hipMemsetAsync(gcnt, 0, 64, s));
+        if (E > 0) {
+            k_group_keys<<<grid_for(E), kBlock, 0, s>>>(g->el_dst, E, kA, gcnt);
+            SSSP_LAUNCH_CHECK("k_group_keys");
+        }
+        int64_t hc[8];
+        SSSP_HIP(hipMemcpyAsync(hc, gcnt, 64, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        int64_t hoff[9], hbase[9];
+        hoff[0] = hbase[0] = 0;
+        for (int x = 0; x < 8; x++) {
+            hbase[x + 1] = hbase[x] + hc[x];
+            hoff[x + 1] = hoff[x] + (hc[x] + 3) / 4 * 4;
+        }
+        g->xgoff_host.assign(hoff, hoff + 9);
+        uint32_t *gperm = nullptr, *gkeys = nullptr;
+        // one radix pass on the 3-bit group key; values = positions in the
+        // src-sorted list (stable)
+        radix_sort_pairs(reinterpret_cast<const int32_t *>(kA), E, 7, kB, kB, vA, vB, counts,
+                         g->blk_a, g->blk_b, g->n_blk, s, &gkeys, &gperm);
+        int64_t *dgo = dmalloc<int64_t>(18);
+        struct F2 {
+            void *p;
+            ~F2() { dfree(p); }
+        } fg2{dgo};
+        SSSP_HIP(hipMemcpyAsync(dgo, hoff, 72, hipMemcpyHostToDevice, s));
+        SSSP_HIP(hipMemcpyAsync(dgo + 9, hbase, 72, hipMemcpyHostToDevice, s));
+        g->x_src = dmalloc<int32_t>((size_t)hoff[8] + 4);
+        g->x_dst = dmalloc<int32_t>((size_t)hoff[8] + 4);
+        g->x_w = dmalloc<uint32_t>((size_t)hoff[8] + 4);
+        g->xgoff = dmalloc<int64_t>(9);
+        SSSP_HIP(hipMemcpyAsync(g->xgoff, hoff, 72, hipMemcpyHostToDevice, s));
+        if (hoff[8] > 0) {
+            k_group_gather<<<grid_for(hoff[8]), kBlock, 0, s>>>(gperm, dgo, dgo + 9, g->el_src,
+                                                                g->el_dst, g->el_w, g->x_src,
+                                                                g->x_dst, g->x_w);
+            SSSP_LAUNCH_CHECK("k_group_gather");
+        }
+        SSSP_HIP(hipStreamSynchronize(s));
+    }
 
     // ---- CSC ----
     if (build_csc) {

@@ -71,6 +71,14 @@ struct sssp_graph {
     int32_t *el_src = nullptr;
     int32_t *el_dst = nullptr;
     uint32_t *el_w = nullptr;
+    // XCD-partitioned edge list: 8 groups by hashed destination block, each
+    // src-sorted and padded to a multiple of 4 edges; group x = edges
+    // [xgoff[x], xgoff[x+1])
+    int32_t *x_src = nullptr;
+    int32_t *x_dst = nullptr;
+    uint32_t *x_w = nullptr;
+    int64_t *xgoff = nullptr;
+    std::vector<int64_t> xgoff_host;
     // CSC: in-edges grouped by dst
     int64_t *col_ptr = nullptr;
     int32_t *csc_src = nullptr;
@@ -115,6 +123,7 @@ struct sssp_graph {
     int max_rounds = 100000;
     double push_edge_frac = 0.05;  // heuristic: sparse push below this frontier-edge fraction
     double pull_frac = 0.5;        // heuristic: pull when unsettled in-edges < pull_frac * frontier edges
+    double delta = 0.0;            // near-far threshold of push rounds (0 = off)
     int trace_events = 0;
 
     // last solve

@@ -233,3 +233,24 @@ def test_autotune_batch_and_pack(gpu, tmp_path):
         assert np.array_equal(bits(a), bits(b))
     finally:
         p.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("delta", [0.01, 0.1, 0.5, 3.0])
+def test_near_far_threshold_keeps_the_fixpoint(gpu, oracle, delta):
+    src, dst, w = S.rmat(14, 16, seed=31, threads=16)
+    V = 1 << 14
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("delta", delta)
+        for s in (0, int(src[123])):
+            ref = oracle.dijkstra(src, dst, w, V, s)
+            for sched in ("push", "heuristic", "learned"):
+                d, st = g.solve(s, sched)
+                assert np.array_equal(bits(d), bits(ref)), (delta, s, sched)
+        ok, counts, _ = g.certify(s)
+        assert ok, counts
+    su, tu, wu = S.er(1 << 13, 8, seed=9, weight_mode=1)
+    with S.DeviceGraph(su, tu, wu, 1 << 13) as g:
+        g.set_param("delta", delta * 100)
+        d, _ = g.solve(0, "push")
+        assert np.array_equal(d, oracle.dijkstra(su, tu, wu, 1 << 13, 0))
