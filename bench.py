@@ -177,7 +177,8 @@ def main():
     pass_ms = float(np.mean(tr["ms"]))
     algo_bytes = 12 * E + 8 * V
     achieved = algo_bytes / (pass_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_el_pass (all-edge pass, src-sorted edge list)",
+    roofline = {"bound": "hbm",
+                "kernel": "k_el_pass (all-edge pass, XCD-partitioned src-sorted edge list)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes_per_pass": algo_bytes, "avg_pass_ms": round(pass_ms, 4),

@@ -144,7 +144,12 @@ int sssp_get_trace(sssp_graph *g, int32_t cap, int32_t *modes, int64_t *frontier
 int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *counts,
                  int32_t *valid);
 /* Tuning knobs by name ("push_chunk", "all_chunk", "pull_chunk",
- * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac"). */
+ * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
+ * "delta", "hub_degree", "trace_events"; "cu_count" read-only).
+ * delta > 0 turns on the far pile of push rounds: a frontier vertex with
+ * d >= (min distance over the frontier and pile) + delta and out-degree >=
+ * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.
+ * Distances do not depend on any knob. */
 int sssp_set_param(sssp_graph *g, const char *name, double value);
 int sssp_get_param(const sssp_graph *g, const char *name, double *value);
 

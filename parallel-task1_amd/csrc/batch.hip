@@ -139,9 +139,14 @@ __global__ __launch_bounds__(BS) void k_wg_solve(const BatchTask *__restrict__ t
                     const int64_t edge = s_start[lo] + (e - s_off[lo]);
                     const int32_t v = adj[edge];
                     const uint32_t cand = WT::add(s_du[lo], wts[edge]);
+                    // rounds share one launch, so a plain read may be from an
+                    // earlier round: the returned old value decides "lowered";
+                    // a stale stamp is an older round and falls through to the
+                    // exchange
                     if (cand < dist[v]) {
                         const uint32_t old = atomicMin(&dist[v], cand);
-                        if (cand < old && atomicExch(&stamp[v - T.vlo], rt) != rt)
+                        int32_t *sp = &stamp[v - T.vlo];
+                        if (cand < old && *sp != rt && atomicExch(sp, rt) != rt)
                             nxt[atomicAdd(&s_n, 1)] = v;
                     }
                 }

@@ -43,7 +43,7 @@ void graph_free(sssp_graph *g)
                     g->queue[0], g->queue[1], g->lb_v, g->lb_start, g->lb_scan, g->lb_tile,
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
-                    g->x_w, g->xgoff};
+                    g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {
@@ -139,7 +139,10 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     const size_t V = (size_t)g->V;
     g->dist = dmalloc<uint32_t>(V);
     g->stamp = dmalloc<int32_t>(V);
-    SSSP_HIP(hipMemsetAsync(g->stamp, 0, 4 * V, s));
+    g->pile[0] = dmalloc<int32_t>(V);
+    g->pile[1] = dmalloc<int32_t>(V);
+    g->farst = dmalloc<int32_t>(V);
+    reset_stamps(g);
     g->queue[0] = dmalloc<int32_t>(V + 1);
     g->queue[1] = dmalloc<int32_t>(V + 1);
     g->lb_v = dmalloc<int32_t>(V + 1);
@@ -165,7 +168,7 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     SSSP_HIP(hipStreamSynchronize(s));
     const int64_t Ei = g->E, Vi = g->V;
     g->device_bytes = Ei * 12 + (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) +
-                      Vi * 4 * 7 + Vi * 16 + Vi * 16;
+                      Vi * 4 * 10 + Vi * 16 + Vi * 16;
 }
 
 struct DevBuf {
@@ -378,6 +381,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         else if (n == "delta") {
             SSSP_REQUIRE(value >= 0, "delta must be >= 0");
             g->delta = value;
+        } else if (n == "hub_degree") {
+            SSSP_REQUIRE(value >= 0 && value <= 2147483647.0, "hub_degree must be in [0, 2^31)");
+            g->hub_degree = (int)value;
         } else if (n == "trace_events") g->trace_events = value != 0;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};
     });
@@ -397,6 +403,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "push_edge_frac") *value = g->push_edge_frac;
         else if (n == "pull_frac") *value = g->pull_frac;
         else if (n == "delta") *value = g->delta;
+        else if (n == "hub_degree") *value = g->hub_degree;
         else if (n == "trace_events") *value = g->trace_events;
         else if (n == "cu_count") *value = g->cu_count;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};

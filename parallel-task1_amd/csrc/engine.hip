@@ -69,48 +69,47 @@ constexpr int kStageCap = 512;  // per-wave LDS staging slots for frontier appen
 // Per-block accumulators of one round, reduced once per block at the end.
 struct RoundAcc {
     int64_t relaxed = 0, qedges = 0;
-    int lowered = 0, reached = 0;
+    int lowered = 0;
     uint32_t minw = 0xFFFFFFFFu;
 };
 
 __device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
 {
     __shared__ int64_t r_rel[kBlock / 64], r_qe[kBlock / 64];
-    __shared__ int r_low[kBlock / 64], r_rea[kBlock / 64];
+    __shared__ int r_low[kBlock / 64];
     __shared__ uint32_t r_min[kBlock / 64];
     const int lane = __lane_id(), w = threadIdx.x >> 6;
     const int64_t rel = wave_sum64(acc.relaxed), qe = wave_sum64(acc.qedges);
-    const int low = wave_sum32(acc.lowered), rea = wave_sum32(acc.reached);
+    const int low = wave_sum32(acc.lowered);
     const uint32_t mn = wave_min_u32(acc.minw);
     if (lane == 0) {
         r_rel[w] = rel;
         r_qe[w] = qe;
         r_low[w] = low;
-        r_rea[w] = rea;
         r_min[w] = mn;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int64_t a = 0, b = 0;
-        int c = 0, d = 0;
+        int c = 0;
         uint32_t m = 0xFFFFFFFFu;
         for (int i = 0; i < kBlock / 64; i++) {
             a += r_rel[i];
             b += r_qe[i];
             c += r_low[i];
-            d += r_rea[i];
             m = r_min[i] < m ? r_min[i] : m;
         }
         if (a) atomicAdd((unsigned long long *)&s->relaxed, (unsigned long long)a);
         if (b) atomicAdd((unsigned long long *)&s->qedges, (unsigned long long)b);
         if (c) atomicAdd(&s->lowered, c);
-        if (d) atomicAdd(&s->reached, d);
         if (m != 0xFFFFFFFFu) atomicMin(&s->minw, m);
     }
 }
 
-// A vertex whose distance this lane lowered joins the next frontier when it
-// has out-edges and no other lane appended it in this round (stamp).
+// A vertex lowered in this round joins the next frontier when it has
+// out-edges and no other lane appended it in this round (stamp). A plain read
+// of the stamp filters sinks (kSinkStamp) and vertices already appended, so
+// the exchange is paid about once per appended vertex.
 template <int DIR>
 __device__ __forceinline__ void append_frontier(bool low, int32_t v, const int64_t *row_ptr,
                                                 int32_t *stamp, int32_t tick_cur, int64_t &qedges,
@@ -118,16 +117,26 @@ __device__ __forceinline__ void append_frontier(bool low, int32_t v, const int64
 {
     bool app = false;
     if (low) {
-        const int64_t deg = row_ptr[v + 1] - row_ptr[v];
-        if (deg > 0) {
-            if constexpr (DIR == DIR_BFS)
-                app = true;
-            else
+        if constexpr (DIR == DIR_BFS) {
+            const int64_t deg = row_ptr[v + 1] - row_ptr[v];
+            app = deg > 0;
+            qedges += deg;
+        } else {
+            const int32_t sv = stamp[v];
+            if (sv != tick_cur && sv != kSinkStamp)
                 app = atomicExch(&stamp[v], tick_cur) != tick_cur;
+            if (app) qedges += row_ptr[v + 1] - row_ptr[v];
         }
-        if (app) qedges += deg;
     }
     stage_push(app, v, wbuf, wcnt);
+}
+
+// Lowers dist[v] to cand without returning the old value. The caller has read
+// dist[v] > cand in this kernel, and distances only decrease, so dist[v] was
+// lowered during the kernel whoever wins: v belongs in the next frontier.
+__device__ __forceinline__ void lower_dist(uint32_t *dist, int32_t v, uint32_t cand)
+{
+    __hip_atomic_fetch_min(&dist[v], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class WT, int DIR, int SRC, int PRED, int K>
@@ -185,28 +194,62 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
             any |= act ? 1 : 0;
         }
         if (!__syncthreads_or(any)) continue;
-        int jj = 0;
-#pragma unroll 1
+        // Three stages so that the K edges of a thread have their loads in
+        // flight together: (1) the entry of each edge (binary search in LDS,
+        // non-decreasing in k), (2) the loads, (3) compare / lower / append.
+        int jk[K];
+        {
+            int jj = 0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int32_t rel = k * kBlock + tid;
+                int lo = jj, hi = m - 1;  // largest jj' with s_off[jj'] <= rel
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= rel) lo = mid;
+                    else hi = mid - 1;
+                }
+                jj = lo;
+                jk[k] = lo;
+            }
+        }
+        uint32_t sk[K];  // push/bfs: d of the edge's source; pull: d of its target (0 = skip)
+        int32_t ok[K];   // the edge's other endpoint
+        uint32_t wk[K];
+        uint32_t dk[K];  // d of the other endpoint
+#pragma unroll
         for (int k = 0; k < K; k++) {
             const int32_t rel = k * kBlock + tid;
             const bool valid = e0 + rel < e1;
-            int lo = jj, hi = m - 1;  // largest jj' with s_off[jj'] <= rel
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= rel) lo = mid;
-                else hi = mid - 1;
+            if constexpr (DIR == DIR_PULL) sk[k] = valid ? s_d[jk[k]] : 0u;
+            else sk[k] = valid ? s_d[jk[k]] : INF;
+            const bool act = (DIR == DIR_PULL) ? sk[k] != 0u : sk[k] != INF;
+            ok[k] = 0;
+            wk[k] = 0;
+            if (act) {
+                const int64_t edge =
+                    (SRC == SRC_DENSE) ? e0 + rel : s_start[jk[k]] + (rel - s_off[jk[k]]);
+                ok[k] = a.adj[edge];
+                wk[k] = a.wts[edge];
             }
-            jj = lo;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const bool act = (DIR == DIR_PULL) ? sk[k] != 0u : sk[k] != INF;
+            dk[k] = act ? a.dist[ok[k]] : INF;
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int32_t rel = k * kBlock + tid;
+            const bool valid = e0 + rel < e1;
+            const int jj = jk[k];
             bool low = false;
             int32_t v = 0;
             if constexpr (DIR == DIR_PULL) {
-                const uint32_t dv = valid ? s_d[jj] : 0u;
+                const uint32_t dv = sk[k];
                 uint32_t cand = INF;
                 if (dv != 0u) {
-                    const int64_t edge = (SRC == SRC_DENSE) ? e0 + rel : s_start[jj] + (rel - s_off[jj]);
-                    const int32_t u = a.adj[edge];
-                    const uint32_t du = a.dist[u];
-                    cand = (du == INF) ? INF : WT::add(du, a.wts[edge]);
+                    cand = (dk[k] == INF) ? INF : WT::add(dk[k], wk[k]);
                     acc.relaxed++;
                 }
                 // segmented min over lanes sharing an entry (jj is non-decreasing in lane)
@@ -221,36 +264,27 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                 const bool head = (lane == 0) || (pseg != seg);
                 if (head && dv != 0u && cand < dv) {
                     v = s_v[jj];
-                    const uint32_t old = atomicMin(&a.dist[v], cand);
-                    if (cand < old) {
-                        low = true;
-                        acc.minw = min(acc.minw, cand);
-                        acc.lowered++;
-                        acc.reached += (old == INF);
-                    }
+                    lower_dist(a.dist, v, cand);
+                    low = true;
+                    acc.minw = min(acc.minw, cand);
+                    acc.lowered++;
                 }
             } else {
-                const uint32_t du = valid ? s_d[jj] : INF;
+                const uint32_t du = sk[k];
                 if (du != INF) {
-                    const int64_t edge = (SRC == SRC_DENSE) ? e0 + rel : s_start[jj] + (rel - s_off[jj]);
-                    v = a.adj[edge];
-                    const uint32_t cand = WT::add(du, a.wts[edge]);
+                    v = ok[k];
+                    const uint32_t cand = WT::add(du, wk[k]);
                     acc.relaxed++;
-                    const uint32_t dv = a.dist[v];
+                    const uint32_t dv = dk[k];
                     if constexpr (DIR == DIR_BFS) {
                         // certificate: tight edge into a finite vertex, first claimer wins
                         if (dv != INF && cand == dv && atomicCAS(&a.parent[v], -1, s_v[jj]) == -1)
                             low = true;
-                    } else {
-                        if (cand < dv) {
-                            const uint32_t old = atomicMin(&a.dist[v], cand);
-                            if (cand < old) {
-                                low = true;
-                                acc.minw = min(acc.minw, cand);
-                                acc.lowered++;
-                                acc.reached += (old == INF);
-                            }
-                        }
+                    } else if (cand < dv) {
+                        lower_dist(a.dist, v, cand);
+                        low = true;
+                        acc.minw = min(acc.minw, cand);
+                        acc.lowered++;
                     }
                 }
             }
@@ -339,13 +373,10 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
                 const uint32_t cand = WT::add(du[i], ew[i]);
                 acc.relaxed++;
                 if (cand < dv[i]) {
-                    const uint32_t old = atomicMin(&dist[v], cand);
-                    if (cand < old) {
-                        low = true;
-                        acc.minw = min(acc.minw, cand);
-                        acc.lowered++;
-                        acc.reached += (old == INF);
-                    }
+                    lower_dist(dist, v, cand);
+                    low = true;
+                    acc.minw = min(acc.minw, cand);
+                    acc.lowered++;
                 }
             }
             append_frontier<DIR_PUSH>(low, v, row_ptr, stamp, tick_cur, acc.qedges, wbuf, wcnt);
@@ -357,18 +388,40 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Sparse-frontier preparation with a near-far split: queue entries whose
-// distance is below `thr` ("near") become load-balancing entries (vertex, row
-// start, exclusive degree scan, tile table); the others ("far") are carried
-// unprocessed into the next frontier, so every vertex with unrelaxed
-// out-edges stays queued. thr = 0xFFFFFFFF disables the split.
+// Sparse-frontier preparation with the far pile. A queue entry is "near"
+// (relaxed in this round) when its out-degree is below hub_degree or its
+// distance is below thr; otherwise it moves to the next far pile and waits
+// until its distance comes within delta of the frontier minimum. Near
+// entries become load-balancing entries (vertex, row start, exclusive degree
+// scan, tile table). Entries still in the pile are skipped (the pile holds
+// them); entries the pile filter moved out this round are near.
+// thr = 0xFFFFFFFF with no pile (farst == nullptr): every entry is near.
 // ---------------------------------------------------------------------------
+struct NearRule {
+    uint32_t thr;
+    int32_t hub_degree;
+    const int32_t *farst;  // nullptr: no pile
+    int32_t tick_cur;
+};
+
+enum { kSkip = 0, kNear = 1, kFar = 2 };
+
+__device__ __forceinline__ int classify(const NearRule &r, int32_t v, uint32_t d, int64_t deg)
+{
+    if (!r.farst) return kNear;
+    const int32_t f = r.farst[v];
+    if (f == kPileIn) return kSkip;
+    if (f == r.tick_cur) return kNear;
+    return (d < r.thr || deg < r.hub_degree) ? kNear : kFar;
+}
+
 __global__ __launch_bounds__(kBlock) void k_lb_reduce(const int32_t *q, int32_t n,
-                                                      const int64_t *row_ptr, const uint32_t *dist,
-                                                      uint32_t thr, int64_t *blk_deg,
-                                                      int64_t *blk_cnt)
+                                                      const int32_t *dev_n, const int64_t *row_ptr,
+                                                      const uint32_t *dist, NearRule rule,
+                                                      int64_t *blk_deg, int64_t *blk_cnt)
 {
     __shared__ int64_t s_warp[kBlock / 64];
+    if (dev_n) n = *dev_n;
     const int64_t base = (int64_t)blockIdx.x * kLbTile + (int64_t)threadIdx.x * kLbItems;
     int64_t s = 0, c = 0;
 #pragma unroll
@@ -376,8 +429,9 @@ __global__ __launch_bounds__(kBlock) void k_lb_reduce(const int32_t *q, int32_t 
         const int64_t j = base + i;
         if (j < n) {
             const int32_t v = q[j];
-            if (dist[v] < thr) {
-                s += row_ptr[v + 1] - row_ptr[v];
+            const int64_t deg = row_ptr[v + 1] - row_ptr[v];
+            if (classify(rule, v, dist[v], deg) == kNear) {
+                s += deg;
                 c++;
             }
         }
@@ -394,9 +448,10 @@ __global__ __launch_bounds__(kBlock) void k_lb_reduce(const int32_t *q, int32_t 
 struct LbWriteArgs {
     const int32_t *q;
     int32_t n;
+    const int32_t *dev_n;    // when set, the entry count is read from the device
     const int64_t *row_ptr;
     const uint32_t *dist;
-    uint32_t thr;
+    NearRule rule;
     const int64_t *blk_deg;  // scanned, [nb] = total edges
     const int64_t *blk_cnt;  // scanned, [nb] = near entries
     int64_t nb;
@@ -405,21 +460,21 @@ struct LbWriteArgs {
     int64_t *lscan;
     int32_t *ltile;
     int64_t TE;
-    Ctrl *ctrl;              // receives n_compact / lb_total
-    int32_t *stamp;          // far carry: dedup against this round's appends
-    int32_t tick_cur;
-    int32_t *qnext;
-    RoundSlot *slot;
+    Ctrl *ctrl;              // receives n_compact / lb_total and the next pile's counters
+    int32_t *farst;          // far entries are marked kPileIn
+    int32_t *pile_next;
+    int nxt;                 // index of the next pile's counters
 };
 
 __global__ __launch_bounds__(kBlock) void k_lb_write(LbWriteArgs a)
 {
     __shared__ int64_t s_warp[kBlock / 64];
+    const int32_t n = a.dev_n ? *a.dev_n : a.n;
     const int64_t base = (int64_t)blockIdx.x * kLbTile + (int64_t)threadIdx.x * kLbItems;
     int32_t vv[kLbItems];
     uint32_t dd[kLbItems];
     int64_t st[kLbItems], dg[kLbItems];
-    bool nr[kLbItems];
+    int cls[kLbItems];
     int64_t s = 0, c = 0;
 #pragma unroll
     for (int i = 0; i < kLbItems; i++) {
@@ -428,15 +483,15 @@ __global__ __launch_bounds__(kBlock) void k_lb_write(LbWriteArgs a)
         dd[i] = 0xFFFFFFFFu;
         st[i] = 0;
         dg[i] = 0;
-        nr[i] = false;
-        if (j < a.n) {
+        cls[i] = kSkip;
+        if (j < n) {
             vv[i] = a.q[j];
             dd[i] = a.dist[vv[i]];
             st[i] = a.row_ptr[vv[i]];
             dg[i] = a.row_ptr[vv[i] + 1] - st[i];
-            nr[i] = dd[i] < a.thr;
+            cls[i] = classify(a.rule, vv[i], dd[i], dg[i]);
         }
-        if (nr[i]) {
+        if (cls[i] == kNear) {
             s += dg[i];
             c++;
         }
@@ -451,11 +506,11 @@ __global__ __launch_bounds__(kBlock) void k_lb_write(LbWriteArgs a)
         a.ctrl->lb_total = total_edges;
         if (total_near == 0) a.ltile[0] = 0;
     }
-    int64_t carried_edges = 0;
-    uint32_t carried_min = 0xFFFFFFFFu;
+    int64_t far_edges = 0;
+    uint32_t far_min = 0xFFFFFFFFu;
 #pragma unroll
     for (int i = 0; i < kLbItems; i++) {
-        if (nr[i]) {
+        if (cls[i] == kNear) {
             const int64_t j = idx;
             a.lv[j] = vv[i];
             a.ls[j] = st[i];
@@ -469,23 +524,80 @@ __global__ __launch_bounds__(kBlock) void k_lb_write(LbWriteArgs a)
                 a.ltile[ntiles] = (int32_t)total_near;
             }
         }
-        // far entries go on to the next frontier (once per round)
-        bool carry = false;
-        if (base + i < a.n && !nr[i]) {
-            carry = atomicExch(&a.stamp[vv[i]], a.tick_cur) != a.tick_cur;
-            if (carry) {
-                carried_edges += dg[i];
-                carried_min = min(carried_min, dd[i]);
+        const bool far = cls[i] == kFar;
+        if (far) {
+            a.farst[vv[i]] = kPileIn;
+            far_edges += dg[i];
+            far_min = min(far_min, dd[i]);
+        }
+        wave_append(far, vv[i], a.pile_next, &a.ctrl->pile_n[a.nxt]);
+    }
+    far_edges = wave_sum64(far_edges);
+    far_min = wave_min_u32(far_min);
+    if (__lane_id() == 0) {
+        if (far_edges)
+            atomicAdd((unsigned long long *)&a.ctrl->pile_edges[a.nxt], (unsigned long long)far_edges);
+        if (far_min != 0xFFFFFFFFu) atomicMin(&a.ctrl->pile_min[a.nxt], far_min);
+    }
+}
+
+// Pile filter of a push round: pile entries whose distance is below thr move
+// out (state = this round's tick) and join the queue unless they are on it
+// already (stamped by the previous round); the others go to the next pile.
+__global__ __launch_bounds__(kBlock) void k_pile_filter(const int32_t *pile, int32_t n,
+                                                        const uint32_t *dist, uint32_t thr,
+                                                        int32_t *farst, const int32_t *stamp,
+                                                        int32_t tick_prev, int32_t tick_cur,
+                                                        int32_t *q, const int64_t *row_ptr,
+                                                        int32_t *pile_next, Ctrl *c, int nxt)
+{
+    int64_t kept_edges = 0;
+    uint32_t kept_min = 0xFFFFFFFFu;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < n;
+         base += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = base + threadIdx.x;
+        bool mv = false, keep = false;
+        int32_t v = 0;
+        if (i < n) {
+            v = pile[i];
+            const uint32_t d = dist[v];
+            if (d < thr) {
+                farst[v] = tick_cur;
+                mv = stamp[v] != tick_prev;
+            } else {
+                keep = true;
+                kept_edges += row_ptr[v + 1] - row_ptr[v];
+                kept_min = min(kept_min, d);
             }
         }
-        wave_append(carry, vv[i], a.qnext, &a.slot->qlen);
+        wave_append(mv, v, q, &c->lb_n);
+        wave_append(keep, v, pile_next, &c->pile_n[nxt]);
     }
-    carried_edges = wave_sum64(carried_edges);
-    carried_min = wave_min_u32(carried_min);
+    kept_edges = wave_sum64(kept_edges);
+    kept_min = wave_min_u32(kept_min);
     if (__lane_id() == 0) {
-        if (carried_edges)
-            atomicAdd((unsigned long long *)&a.slot->qedges, (unsigned long long)carried_edges);
-        if (carried_min != 0xFFFFFFFFu) atomicMin(&a.slot->minw, carried_min);
+        if (kept_edges)
+            atomicAdd((unsigned long long *)&c->pile_edges[nxt], (unsigned long long)kept_edges);
+        if (kept_min != 0xFFFFFFFFu) atomicMin(&c->pile_min[nxt], kept_min);
+    }
+}
+
+// Before an all-edge or pull round: every piled vertex becomes active for
+// this round (stamp of the consumed frontier) and leaves the pile.
+__global__ __launch_bounds__(kBlock) void k_pile_flush(const int32_t *pile, int32_t n,
+                                                       int32_t *stamp, int32_t tick_prev,
+                                                       int32_t *farst, Ctrl *c, int cur)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int32_t v = pile[i];
+        stamp[v] = tick_prev;
+        farst[v] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->pile_n[cur] = 0;
+        c->pile_min[cur] = 0xFFFFFFFFu;
+        c->pile_edges[cur] = 0;
     }
 }
 
@@ -500,6 +612,26 @@ __global__ __launch_bounds__(kBlock) void k_fill(T *p, int64_t n, T val)
         p[i] = val;
 }
 
+__global__ __launch_bounds__(kBlock) void k_stamp_init(int32_t *stamp, const int64_t *row_ptr,
+                                                       int64_t V)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * kBlock)
+        stamp[v] = row_ptr[v + 1] > row_ptr[v] ? 0 : kSinkStamp;
+}
+
+template <class WT>
+__global__ __launch_bounds__(kBlock) void k_count_reached(const uint32_t *dist, int64_t V,
+                                                          int64_t *out)
+{
+    int64_t n = 0;
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * kBlock)
+        n += dist[v] != WT::INF;
+    n = wave_sum64(n);
+    if (__lane_id() == 0 && n) atomicAdd((unsigned long long *)out, (unsigned long long)n);
+}
+
 __device__ void zero_slot(RoundSlot *s, uint32_t minw)
 {
     s->qlen = 0;
@@ -507,7 +639,7 @@ __device__ void zero_slot(RoundSlot *s, uint32_t minw)
     s->qedges = 0;
     s->relaxed = 0;
     s->lowered = 0;
-    s->reached = 0;
+    s->reserved0 = 0;
     s->sample_unsettled = 0;
     s->sample_n = 0;
     s->sample_unsettled_in = 0;
@@ -522,7 +654,14 @@ __global__ void k_init_solve(Ctrl *c, uint32_t *dist, int32_t *stamp, int32_t *q
     zero_slot(&c->slot[1], 0xFFFFFFFFu);
     for (int i = 0; i < 64; i++) c->changed[i] = 0;
     for (int i = 0; i < 8; i++) c->cert[i] = 0;
+    c->n_reached = 0;
     c->err = 0;
+    for (int i = 0; i < 2; i++) {
+        c->pile_n[i] = 0;
+        c->pile_min[i] = 0xFFFFFFFFu;
+        c->pile_edges[i] = 0;
+    }
+    c->lb_n = 0;
     if (source < 0) return;
     dist[source] = 0u;
     const int64_t deg = row_ptr[source + 1] - row_ptr[source];
@@ -553,8 +692,9 @@ __global__ __launch_bounds__(kBlock) void k_init_sources(Ctrl *c, uint32_t *dist
 }
 
 // Zeroes the slot the coming round fills and samples the unsettled fraction
-// of the frontier it consumes (d > minF, minF = min distance over the frontier).
-__global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r,
+// of the frontier it consumes (d > minF, minF = min distance over the frontier
+// and, pile >= 0, over the far pile).
+__global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r, int pile,
                                                         const int32_t *sample_v, int32_t n_sample,
                                                         const uint32_t *dist,
                                                         const int64_t *col_ptr)
@@ -565,7 +705,7 @@ __global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r,
         zero_slot(nxt, 0xFFFFFFFFu);
         cur->sample_n = n_sample;
     }
-    const uint32_t minF = cur->minw;
+    const uint32_t minF = pile >= 0 ? min(cur->minw, c->pile_min[pile]) : cur->minw;
     int cnt = 0;
     int64_t ind = 0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_sample;
@@ -593,6 +733,7 @@ struct DecideArgs {
     float pull_frac;
     int push_chunk, all_chunk, pull_chunk;
     const float *mlp;  // nullable
+    int pile;          // pile buffer the round reads
 };
 
 // One wave: features, the hand-written rule and (learned schedule) the MLP.
@@ -604,7 +745,9 @@ __global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
     __shared__ float s_o[kOut];
     const int j = threadIdx.x;
     const RoundSlot *cur = &c->slot[d.r & 1];
-    const double fv = cur->qlen, fe = (double)cur->qedges;
+    // the frontier is the queue plus the far pile
+    const double fv = (double)cur->qlen + c->pile_n[d.pile];
+    const double fe = (double)cur->qedges + (double)c->pile_edges[d.pile];
     const double sn = cur->sample_n > 0 ? cur->sample_n : 1;
     const double uns = cur->sample_unsettled / sn;
     const double uns_in = (double)cur->sample_unsettled_in * (d.V / sn);
@@ -662,6 +805,13 @@ __global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
         }
     }
     if (j == 0) {
+        // counters a push round fills: the next pile and the queue length
+        // after the pile filter
+        const int nx = d.pile ^ 1;
+        c->pile_n[nx] = 0;
+        c->pile_min[nx] = 0xFFFFFFFFu;
+        c->pile_edges[nx] = 0;
+        c->lb_n = cur->qlen;
         c->dec_mode = mode;
         c->dec_chunk = chunk;
         c->dec_mlp = by_mlp;
@@ -695,8 +845,10 @@ __global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ 
         if (du == WT::INF) return;
         const DT cand = WT::add(du, ww);
         if (cand < dv) {
-            const DT old = atomicMin(&dist[t], cand);
-            low |= cand < old;
+            // dv was read in this pass: the distance drops in this pass
+            // whoever wins, so the result of the min is not needed
+            __hip_atomic_fetch_min(&dist[t], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            low = true;
         }
     };
     if constexpr (VEC == 4) {
@@ -878,21 +1030,24 @@ uint32_t near_threshold(int wtype, uint32_t minF, double delta)
     return thr > minF ? thr : minF + 1;
 }
 
-// Builds the near entries of a sparse frontier (queue with out-degree > 0
-// only) and carries the far ones into the next queue.
-void lb_prepare(sssp_graph *g, const int32_t *q, int32_t n, int kidx, uint32_t thr,
-                int32_t tick_cur, int32_t *qnext, RoundSlot *slot_next)
+// Builds the near entries of a sparse frontier q[0, n) -- n is an upper
+// bound when dev_n is set (pile entries moved onto the queue) -- and, with a
+// pile in the rule, moves far entries to pile buffer `nxt`.
+void lb_prepare(sssp_graph *g, const int32_t *q, int32_t n, const int32_t *dev_n, int kidx,
+                const NearRule &rule, int nxt)
 {
     hipStream_t s = g->stream;
-    const int64_t nb = (n + kLbTile - 1) / kLbTile;
+    int64_t nb = (n + kLbTile - 1) / kLbTile;
+    if (nb < 1) nb = 1;
     if (nb + 1 > g->n_blk) throw Error{SSSP_ERR_INTERNAL, "lb scratch too small"};
     const int64_t TE = (int64_t)kBlock * kChunkK[kidx];
-    k_lb_reduce<<<(unsigned)nb, kBlock, 0, s>>>(q, n, g->row_ptr, g->dist, thr, g->blk_a, g->blk_b);
+    k_lb_reduce<<<(unsigned)nb, kBlock, 0, s>>>(q, n, dev_n, g->row_ptr, g->dist, rule, g->blk_a,
+                                                g->blk_b);
     SSSP_LAUNCH_CHECK("k_lb_reduce");
     scan_blocks_inplace(g->blk_a, nb, s);
     scan_blocks_inplace(g->blk_b, nb, s);
-    LbWriteArgs w{q, n, g->row_ptr, g->dist, thr, g->blk_a, g->blk_b, nb, g->lb_v, g->lb_start,
-                  g->lb_scan, g->lb_tile, TE, g->ctrl, g->stamp, tick_cur, qnext, slot_next};
+    LbWriteArgs w{q, n, dev_n, g->row_ptr, g->dist, rule, g->blk_a, g->blk_b, nb, g->lb_v,
+                  g->lb_start, g->lb_scan, g->lb_tile, TE, g->ctrl, g->farst, g->pile[nxt], nxt};
     k_lb_write<<<(unsigned)nb, kBlock, 0, s>>>(w);
     SSSP_LAUNCH_CHECK("k_lb_write");
 }
@@ -912,16 +1067,26 @@ AdvArgs base_args(sssp_graph *g, int32_t r, int32_t tick0)
     return a;
 }
 
-// Runs one relaxation round in the given mode.
+// Runs one relaxation round in the given mode. `pile` is the pile buffer the
+// round reads; the host mirror of the control block holds its counters.
+// Returns true when a push round wrote the next pile (the caller swaps).
 template <class WT>
-void run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, const RoundSlot &cur,
+bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, const RoundSlot &cur,
                bool verify)
 {
     AdvArgs a = base_args<WT>(g, r, tick0);
+    hipStream_t s = g->stream;
+    const int p = g->pile_cur;
+    const Ctrl &h = *g->ctrl_host;
+    const int32_t pn = h.pile_n[p];
+    if (mode != SSSP_MODE_PUSH && !verify && pn > 0) {
+        k_pile_flush<<<grid_for(pn, 4096), kBlock, 0, s>>>(g->pile[p], pn, g->stamp, a.tick_prev,
+                                                           g->farst, g->ctrl, p);
+        SSSP_LAUNCH_CHECK("k_pile_flush");
+    }
     if (verify || mode == SSSP_MODE_ALL) {
         // all-edge round streamed over the src-sorted edge list; the chunk
         // bucket sets the grid (2, 4, 8 or 16 workgroups per CU)
-        hipStream_t s = g->stream;
         const int64_t nblocks = (int64_t)g->cu_count * (2 << kidx);
         if (verify)
             k_el_active<WT, PRED_REACHED><<<(unsigned)nblocks, kBlock, 0, s>>>(
@@ -932,25 +1097,42 @@ void run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
                 g->x_src, g->x_dst, g->x_w, g->E, g->dist, g->stamp, a.tick_prev, a.tick_cur,
                 g->row_ptr, a.qnext, a.slot, g->xgoff);
         SSSP_LAUNCH_CHECK("k_el_active");
-        return;
+        return false;
     }
     switch (mode) {
     case SSSP_MODE_PUSH: {
-        const int32_t *q = g->queue[r & 1];
-        const uint32_t thr = near_threshold(g->wtype, cur.minw, g->delta);
-        lb_prepare(g, q, cur.qlen, kidx, thr, a.tick_cur, a.qnext, a.slot);
+        int32_t *q = g->queue[r & 1];
+        const bool piled = g->delta > 0;
+        NearRule rule{0xFFFFFFFFu, 0, nullptr, a.tick_cur};
+        int32_t nbound = cur.qlen;
+        int64_t ebound = cur.qedges;
+        if (piled) {
+            const uint32_t minF = cur.minw < h.pile_min[p] ? cur.minw : h.pile_min[p];
+            rule.thr = near_threshold(g->wtype, minF, g->delta);
+            rule.hub_degree = g->hub_degree;
+            rule.farst = g->farst;
+            if (pn > 0) {
+                k_pile_filter<<<grid_for(pn, 4096), kBlock, 0, s>>>(
+                    g->pile[p], pn, g->dist, rule.thr, g->farst, g->stamp, a.tick_prev, a.tick_cur,
+                    q, g->row_ptr, g->pile[p ^ 1], g->ctrl, p ^ 1);
+                SSSP_LAUNCH_CHECK("k_pile_filter");
+                nbound += pn;
+                ebound += h.pile_edges[p];
+            }
+        }
+        lb_prepare(g, q, nbound, piled ? &g->ctrl->lb_n : nullptr, kidx, rule, p ^ 1);
         a.ent_v = g->lb_v;
         a.ent_start = g->lb_start;
         a.ent_scan = g->lb_scan;
         a.tile = g->lb_tile;
-        a.n = cur.qlen;     // upper bounds for the grid; the kernel reads the
-        a.T = cur.qedges;   // compacted counts from the device
+        a.n = nbound;       // upper bounds for the grid; the kernel reads the
+        a.T = ebound;       // compacted counts from the device
         a.dev_n = &g->ctrl->n_compact;
         a.dev_T = &g->ctrl->lb_total;
         a.adj = g->el_dst;
         a.wts = g->el_w;
         launch_advance<WT>(g, Kind::SparsePush, kidx, a);
-        break;
+        return piled;
     }
     case SSSP_MODE_PULL: {
         a.ent_v = g->nzi_v;
@@ -960,9 +1142,10 @@ void run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
         a.T = g->E;
         a.adj = g->csc_src;
         a.wts = g->csc_w;
-        a.minF = cur.minw;
+        // settled test against the minimum over the frontier and the pile
+        a.minF = cur.minw < h.pile_min[p] ? cur.minw : h.pile_min[p];
         launch_advance<WT>(g, Kind::DensePull, kidx, a);
-        break;
+        return false;
     }
     default: throw Error{SSSP_ERR_INTERNAL, "bad round mode"};
     }
@@ -1039,12 +1222,12 @@ void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_
     const int ns = g->n_sample;
     bool verified = false;
     int64_t relaxed = 0;
-    int32_t reached = 1;
+    g->pile_cur = 0;
     for (int32_t r = 0;; r++) {
         if (r >= g->max_rounds)
             throw Error{SSSP_ERR_NOT_CONVERGED, "solve did not converge within max_rounds"};
-        k_round_begin<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->sample_v, ns, g->dist,
-                                                          g->has_csc ? g->col_ptr : nullptr);
+        k_round_begin<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v, ns,
+                                                          g->dist, g->has_csc ? g->col_ptr : nullptr);
         SSSP_LAUNCH_CHECK("k_round_begin");
         DecideArgs da{};
         da.r = r;
@@ -1058,14 +1241,16 @@ void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_
         da.all_chunk = g->all_chunk;
         da.pull_chunk = g->pull_chunk;
         da.mlp = (learned && g->mlp) ? mlp_device_params(g->mlp) : nullptr;
+        da.pile = g->pile_cur;
         k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
         SSSP_LAUNCH_CHECK("k_decide");
         read_slot(g, r);
         const RoundSlot cur = g->ctrl_host->slot[r & 1];
+        const int32_t piled = g->ctrl_host->pile_n[g->pile_cur];
+        const int64_t piled_edges = g->ctrl_host->pile_edges[g->pile_cur];
         relaxed += cur.relaxed;
-        reached += cur.reached;
         bool verify = false;
-        if (cur.qlen == 0) {
+        if (cur.qlen == 0 && piled == 0) {
             if (!g->verify || verified) break;
             verify = true;
         }
@@ -1085,6 +1270,11 @@ void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_
         }
         if (mode == SSSP_MODE_PULL && !g->has_csc) mode = SSSP_MODE_ALL;
         if (kidx < 0 || kidx >= kChunkCount) kidx = 1;
+        if (!verify && cur.qlen == 0) {
+            // only the pile is pending: a push round drains it
+            mode = SSSP_MODE_PUSH;
+            kidx = g->push_chunk;
+        }
         if (verify) {
             mode = SSSP_MODE_ALL;
             kidx = g->all_chunk;
@@ -1094,7 +1284,7 @@ void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_
             verified = false;
         }
         if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
-        run_round<WT>(g, mode, kidx, r, tick0, cur, verify);
+        if (run_round<WT>(g, mode, kidx, r, tick0, cur, verify)) g->pile_cur ^= 1;
         float ms = 0.f;
         if (g->trace_events) {
             SSSP_HIP(hipEventRecord(g->ev_r1, s));
@@ -1104,8 +1294,8 @@ void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_
         TraceRow tr{};
         tr.mode = verify ? 3 : mode;
         tr.chunk = kidx;
-        tr.frontier = cur.qlen;
-        tr.frontier_edges = verify ? g->E : cur.qedges;
+        tr.frontier = (int64_t)cur.qlen + piled;
+        tr.frontier_edges = verify ? g->E : cur.qedges + piled_edges;
         tr.ms = ms;
         tr.unsettled = cur.sample_n > 0 ? (float)cur.sample_unsettled / cur.sample_n : 0.f;
         g->trace.push_back(tr);
@@ -1115,7 +1305,6 @@ void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_
         else st->n_pull++;
     }
     st->edges_relaxed = relaxed;
-    st->reached = reached;
 }
 
 void upload_sources(sssp_graph *g, const int32_t *sources, int32_t n)
@@ -1134,8 +1323,7 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
 {
     hipStream_t s = g->stream;
     if (g->tick > (1 << 30) - g->max_rounds - 4) {
-        k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->stamp, g->V, 0);
-        SSSP_LAUNCH_CHECK("stamp reset");
+        reset_stamps(g);
         g->tick = 1;
     }
     const int32_t tick0 = g->tick;
@@ -1157,7 +1345,19 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
         solve_plain<WT>(g, st);
         g->tick = tick0 + 1;
     } else {
-        solve_rounds<WT>(g, source, sched, tick0, st);
+        if (g->pile_dirty) {
+            // an earlier solve stopped with vertices still piled
+            k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->farst, g->V, 0);
+            SSSP_LAUNCH_CHECK("pile state reset");
+        }
+        g->pile_dirty = true;
+        try {
+            solve_rounds<WT>(g, source, sched, tick0, st);
+        } catch (...) {
+            g->tick = tick0 + g->max_rounds + 2;
+            throw;
+        }
+        g->pile_dirty = false;
         g->tick = tick0 + st->iterations + 2;
     }
     SSSP_HIP(hipEventRecord(g->ev_b, s));
@@ -1166,10 +1366,13 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
     SSSP_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
     st->t_solve_ms = ms;
     st->gteps = ms > 0 ? (double)g->E / (ms * 1e-3) / 1e9 : 0.0;
-    if (sched == SSSP_SCHED_PLAIN) {
-        // reached count for the plain schedule
-        st->reached = -1;
-    }
+    // reached vertices, counted after the timed region
+    k_count_reached<WT><<<grid_for(g->V, 2048), kBlock, 0, s>>>(g->dist, g->V, &g->ctrl->n_reached);
+    SSSP_LAUNCH_CHECK("k_count_reached");
+    int64_t nr = 0;
+    SSSP_HIP(hipMemcpyAsync(&nr, &g->ctrl->n_reached, sizeof(nr), hipMemcpyDeviceToHost, s));
+    SSSP_HIP(hipStreamSynchronize(s));
+    st->reached = (int32_t)nr;
 }
 
 template <class WT>
@@ -1190,13 +1393,13 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
     SSSP_LAUNCH_CHECK("k_cert_roots");
     for (int32_t r = 0;; r++) {
         if (r > g->V + 2) throw Error{SSSP_ERR_INTERNAL, "certificate BFS did not terminate"};
-        k_round_begin<<<1, kBlock, 0, s>>>(c, r, g->sample_v, 0, g->dist, nullptr);
+        k_round_begin<<<1, kBlock, 0, s>>>(c, r, -1, g->sample_v, 0, g->dist, nullptr);
         SSSP_LAUNCH_CHECK("k_round_begin (bfs)");
         read_slot(g, r);
         const RoundSlot cur = g->ctrl_host->slot[r & 1];
         if (cur.qlen == 0) break;
         AdvArgs a = base_args<WT>(g, r, 0);
-        lb_prepare(g, g->queue[r & 1], cur.qlen, 2, 0xFFFFFFFFu, a.tick_cur, a.qnext, a.slot);
+        lb_prepare(g, g->queue[r & 1], cur.qlen, nullptr, 2, NearRule{0xFFFFFFFFu, 0, nullptr, 0}, 0);
         a.ent_v = g->lb_v;
         a.ent_start = g->lb_start;
         a.ent_scan = g->lb_scan;
@@ -1215,44 +1418,72 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
     // with a parent + the source
 }
 
-__global__ void k_zero_slot(Ctrl *c, int32_t which)
+// Restores the control block's round state before a timed try: the slot the
+// round fills, the pile it reads (host values) and the counters it fills.
+__global__ void k_restore_round(Ctrl *c, int32_t r, int p, int32_t qlen, int32_t pn, uint32_t pmin,
+                                int64_t pedges)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) zero_slot(&c->slot[which & 1], 0xFFFFFFFFu);
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    zero_slot(&c->slot[(r + 1) & 1], 0xFFFFFFFFu);
+    c->pile_n[p] = pn;
+    c->pile_min[p] = pmin;
+    c->pile_edges[p] = pedges;
+    c->pile_n[p ^ 1] = 0;
+    c->pile_min[p ^ 1] = 0xFFFFFFFFu;
+    c->pile_edges[p ^ 1] = 0;
+    c->lb_n = qlen;
 }
 
 // Timing collection: before every round, each mode x chunk bucket is run from
-// the same saved state (distances and stamps restored, next slot cleared) and
-// timed with device events; then the fastest is run for real.
+// the same saved state (distances, stamps and pile states restored, control
+// block restored) and timed with device events; then the fastest is run for
+// real.
 template <class WT>
 void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
                int32_t *rows)
 {
     hipStream_t s = g->stream;
     if (g->tick > (1 << 30) - g->max_rounds - 4) {
-        k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->stamp, g->V, 0);
+        reset_stamps(g);
         g->tick = 1;
+    }
+    if (g->pile_dirty) {
+        k_fill<int32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->farst, g->V, 0);
+        SSSP_LAUNCH_CHECK("pile state reset");
     }
     const int32_t tick0 = g->tick;
     const size_t V = (size_t)g->V;
     uint32_t *sd = dmalloc<uint32_t>(V);
-    int32_t *sst = dmalloc<int32_t>(V);
+    int32_t *sst = nullptr, *sfa = nullptr;
     struct Free {
-        void *a, *b;
+        void *a = nullptr, *b = nullptr, *c = nullptr;
         ~Free()
         {
             dfree(a);
             dfree(b);
+            dfree(c);
         }
-    } fr{sd, sst};
+    } fr;
+    fr.a = sd;
+    sst = dmalloc<int32_t>(V);
+    fr.b = sst;
+    sfa = dmalloc<int32_t>(V);
+    fr.c = sfa;
+    g->pile_dirty = true;
+    g->pile_cur = 0;
     k_fill<uint32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->V, WT::INF);
     k_init_solve<<<1, 64, 0, s>>>(g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, source, tick0);
     SSSP_LAUNCH_CHECK("collect init");
     const int nt = kModes * kChunkCount;
     int32_t row = 0, r = 0;
     for (;; r++) {
-        if (r >= g->max_rounds) throw Error{SSSP_ERR_NOT_CONVERGED, "collect: too many rounds"};
+        if (r >= g->max_rounds) {
+            g->tick = tick0 + g->max_rounds + 2;
+            throw Error{SSSP_ERR_NOT_CONVERGED, "collect: too many rounds"};
+        }
+        const int p = g->pile_cur;
         k_round_begin<<<grid_for(g->n_sample, 64), kBlock, 0, s>>>(
-            g->ctrl, r, g->sample_v, g->n_sample, g->dist, g->has_csc ? g->col_ptr : nullptr);
+            g->ctrl, r, p, g->sample_v, g->n_sample, g->dist, g->has_csc ? g->col_ptr : nullptr);
         DecideArgs da{};
         da.r = r;
         da.sched = SSSP_SCHED_HEURISTIC;
@@ -1261,13 +1492,25 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
         da.has_csc = g->has_csc;
         da.push_edge_frac = (float)g->push_edge_frac;
         da.pull_frac = (float)g->pull_frac;
+        da.pile = p;
         k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
         SSSP_LAUNCH_CHECK("collect decide");
         read_slot(g, r);
         const RoundSlot cur = g->ctrl_host->slot[r & 1];
-        if (cur.qlen == 0) break;
+        const int32_t pn = g->ctrl_host->pile_n[p];
+        const uint32_t pmin = g->ctrl_host->pile_min[p];
+        const int64_t pe = g->ctrl_host->pile_edges[p];
+        if (cur.qlen == 0 && pn == 0) break;
         SSSP_HIP(hipMemcpyAsync(sd, g->dist, 4 * V, hipMemcpyDeviceToDevice, s));
         SSSP_HIP(hipMemcpyAsync(sst, g->stamp, 4 * V, hipMemcpyDeviceToDevice, s));
+        SSSP_HIP(hipMemcpyAsync(sfa, g->farst, 4 * V, hipMemcpyDeviceToDevice, s));
+        auto restore = [&] {
+            SSSP_HIP(hipMemcpyAsync(g->dist, sd, 4 * V, hipMemcpyDeviceToDevice, s));
+            SSSP_HIP(hipMemcpyAsync(g->stamp, sst, 4 * V, hipMemcpyDeviceToDevice, s));
+            SSSP_HIP(hipMemcpyAsync(g->farst, sfa, 4 * V, hipMemcpyDeviceToDevice, s));
+            k_restore_round<<<1, 64, 0, s>>>(g->ctrl, r, p, cur.qlen, pn, pmin, pe);
+            SSSP_LAUNCH_CHECK("k_restore_round");
+        };
         float t[kModes * kChunkCount];
         float best = 3.0e38f;
         int bm = 0, bc = 0;
@@ -1275,11 +1518,9 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
             for (int c = 0; c < kChunkCount; c++) {
                 float ms = -1.f;
                 if (!(mode == SSSP_MODE_PULL && !g->has_csc)) {
-                    SSSP_HIP(hipMemcpyAsync(g->dist, sd, 4 * V, hipMemcpyDeviceToDevice, s));
-                    SSSP_HIP(hipMemcpyAsync(g->stamp, sst, 4 * V, hipMemcpyDeviceToDevice, s));
-                    k_zero_slot<<<1, 64, 0, s>>>(g->ctrl, r + 1);
+                    restore();
                     SSSP_HIP(hipEventRecord(g->ev_r0, s));
-                    run_round<WT>(g, mode, c, r, tick0, cur, false);
+                    (void)run_round<WT>(g, mode, c, r, tick0, cur, false);
                     SSSP_HIP(hipEventRecord(g->ev_r1, s));
                     SSSP_HIP(hipEventSynchronize(g->ev_r1));
                     SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
@@ -1292,10 +1533,8 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
                 t[mode * kChunkCount + c] = ms;
             }
         }
-        SSSP_HIP(hipMemcpyAsync(g->dist, sd, 4 * V, hipMemcpyDeviceToDevice, s));
-        SSSP_HIP(hipMemcpyAsync(g->stamp, sst, 4 * V, hipMemcpyDeviceToDevice, s));
-        k_zero_slot<<<1, 64, 0, s>>>(g->ctrl, r + 1);
-        run_round<WT>(g, bm, bc, r, tick0, cur, false);
+        restore();
+        if (run_round<WT>(g, bm, bc, r, tick0, cur, false)) g->pile_cur ^= 1;
         if (row < cap) {
             for (int i = 0; i < kIn; i++) feats[(size_t)row * kIn + i] = g->ctrl_host->feat[i];
             for (int i = 0; i < nt; i++) times[(size_t)row * nt + i] = t[i];
@@ -1303,11 +1542,21 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
         }
     }
     SSSP_HIP(hipStreamSynchronize(s));
+    g->pile_dirty = false;
     g->tick = tick0 + r + 2;
     *rows = row;
 }
 
 }  // namespace
+
+void reset_stamps(sssp_graph *g)
+{
+    k_stamp_init<<<grid_for(g->V), kBlock, 0, g->stream>>>(g->stamp, g->row_ptr, g->V);
+    SSSP_LAUNCH_CHECK("k_stamp_init");
+    k_fill<int32_t><<<grid_for(g->V), kBlock, 0, g->stream>>>(g->farst, g->V, 0);
+    SSSP_LAUNCH_CHECK("pile state reset");
+    g->pile_dirty = false;
+}
 
 void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp_stats *st)
 {

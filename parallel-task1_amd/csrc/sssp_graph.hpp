@@ -11,6 +11,12 @@ constexpr int kMaxRounds = 1 << 16;  // hard cap on relaxation rounds per solve
 constexpr int kChunkCount = SSSP_MLP_CHUNKS;
 // chunk width bucket -> edges per lane per tile (tile = 256 * K edges)
 constexpr int kChunkK[kChunkCount] = {1, 2, 4, 8};
+// Frontier stamp of a vertex without out-edges: above every round tick (ticks
+// restart below 2^30), so such a vertex is never appended to a frontier.
+constexpr int32_t kSinkStamp = 0x7FFFFFFF;
+// Pile state of a vertex: kPileIn while it waits in the far pile, else the
+// tick of the round that last moved it out (0 = never).
+constexpr int32_t kPileIn = -1;
 
 // Per-round counters written by the relaxation kernels. Two slots indexed by
 // round parity: round r writes slot (r+1)&1 (the frontier it builds) while
@@ -20,8 +26,8 @@ struct alignas(64) RoundSlot {
     uint32_t minw;         // min distance written this round (bit pattern)
     int64_t qedges;        // sum of out-degrees of appended vertices
     int64_t relaxed;       // edge relaxations evaluated
-    int32_t lowered;       // successful atomicMin lowerings
-    int32_t reached;       // first reaches (INF -> finite)
+    int32_t lowered;       // relaxations whose candidate beat the distance read
+    int32_t reserved0;
     // filled by the round-begin kernel for the consuming round:
     int32_t sample_unsettled;   // sampled vertices with d > minF
     int32_t sample_n;           // sample size
@@ -34,6 +40,7 @@ struct Ctrl {
     int64_t n_compact;     // lb scan: entries
     int64_t lb_total;      // lb scan: total edges
     int64_t cert[8];       // certificate counters
+    int64_t n_reached;     // vertices with a finite distance after a solve
     int32_t err;           // device-side error flag
     // per-round decision written by the decision kernel
     int32_t dec_mode;
@@ -42,6 +49,12 @@ struct Ctrl {
     int32_t dec_heur_mode;
     float feat[SSSP_MLP_IN];
     float logits[SSSP_MLP_OUT];
+    // far pile of push rounds (double-buffered): frontier vertices whose
+    // relaxation is deferred until their distance is near the frontier minimum
+    int32_t pile_n[2];
+    uint32_t pile_min[2];     // min distance over the pile when it was built
+    int64_t pile_edges[2];    // out-edges of the pile's vertices
+    int32_t lb_n;             // queue entries plus pile entries moved onto the queue
 };
 
 struct TraceRow {
@@ -111,6 +124,10 @@ struct sssp_graph {
     int64_t *blk_b = nullptr;
     int64_t n_blk = 0;
     int32_t *parent = nullptr;    // certificate tight tree
+    int32_t *pile[2] = {nullptr, nullptr};  // far pile (push rounds), double-buffered
+    int32_t *farst = nullptr;     // per-vertex pile state (kPileIn / tick moved out)
+    int pile_cur = 0;             // pile buffer the coming round reads
+    bool pile_dirty = false;      // a solve stopped with vertices still piled
     sssp::Ctrl *ctrl = nullptr;
     sssp::Ctrl *ctrl_host = nullptr;  // pinned mirror
     int32_t tick = 1;
@@ -123,7 +140,8 @@ struct sssp_graph {
     int max_rounds = 100000;
     double push_edge_frac = 0.05;  // heuristic: sparse push below this frontier-edge fraction
     double pull_frac = 0.5;        // heuristic: pull when unsettled in-edges < pull_frac * frontier edges
-    double delta = 0.0;            // near-far threshold of push rounds (0 = off)
+    double delta = 0.0;            // push rounds defer vertices with d >= min + delta (0 = off)
+    int hub_degree = 0;            // ... unless their out-degree is below this
     int trace_events = 0;
 
     // last solve
@@ -203,6 +221,7 @@ void ref_init(DT *d, int32_t V, int32_t source);
 template <class WT, class DT>
 void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int64_t E, DT *d,
             int32_t *passes_out);
+void reset_stamps(sssp_graph *g);  // stamps: 0 (out-edges) / kSinkStamp; pile states: 0
 void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp_stats *st);
 void certify(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *counts);
 void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,

@@ -74,6 +74,7 @@ def test_rmat_every_schedule_bit_exact(gpu, oracle, scale, ef, seed, threads):
             for sched in SCHEDS:
                 d, st = g.solve(s, sched)
                 assert np.array_equal(bits(d), bits(ref)), (s, sched, st)
+                assert st["reached"] == int((d != S.F32_INF).sum()), (s, sched, st)
             ok, counts, par = g.certify(s)
             assert ok, counts
             rc, oc = oracle.certify(src, dst, w, V, s, d, par)
@@ -93,6 +94,7 @@ def test_er_u32_bit_exact(gpu, oracle, logV, deg):
             d, st = g.solve(0, sched)
             assert d.dtype == np.uint32
             assert np.array_equal(d, ref), sched
+            assert st["reached"] == int((d != np.uint32(0xFFFFFFFF)).sum()), (sched, st)
         ok, counts, par = g.certify(0)
         assert ok, counts
         rc, oc = oracle.certify(src, dst, w, V, 0, d, par)
@@ -236,21 +238,49 @@ def test_autotune_batch_and_pack(gpu, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("delta", [0.01, 0.1, 0.5, 3.0])
-def test_near_far_threshold_keeps_the_fixpoint(gpu, oracle, delta):
+@pytest.mark.parametrize("delta,hub", [(0.01, 0), (0.1, 0), (0.5, 0), (3.0, 0), (0.005, 64),
+                                       (0.02, 16)])
+def test_far_pile_keeps_the_fixpoint(gpu, oracle, delta, hub):
+    """Push rounds defer vertices (d >= min + delta, out-degree >= hub) to the
+    far pile; mixed schedules flush it before all-edge and pull rounds."""
     src, dst, w = S.rmat(14, 16, seed=31, threads=16)
     V = 1 << 14
     with S.DeviceGraph(src, dst, w, V) as g:
         g.set_param("delta", delta)
+        g.set_param("hub_degree", hub)
         for s in (0, int(src[123])):
             ref = oracle.dijkstra(src, dst, w, V, s)
-            for sched in ("push", "heuristic", "learned"):
+            for sched in ("push", "heuristic", "learned", "all", "pull"):
                 d, st = g.solve(s, sched)
-                assert np.array_equal(bits(d), bits(ref)), (delta, s, sched)
+                assert np.array_equal(bits(d), bits(ref)), (delta, hub, s, sched)
+                assert st["reached"] == int((d != S.F32_INF).sum())
         ok, counts, _ = g.certify(s)
         assert ok, counts
     su, tu, wu = S.er(1 << 13, 8, seed=9, weight_mode=1)
     with S.DeviceGraph(su, tu, wu, 1 << 13) as g:
         g.set_param("delta", delta * 100)
+        g.set_param("hub_degree", hub)
         d, _ = g.solve(0, "push")
         assert np.array_equal(d, oracle.dijkstra(su, tu, wu, 1 << 13, 0))
+
+
+@pytest.mark.gpu
+def test_far_pile_after_failed_solve_and_collection(gpu, oracle):
+    """A solve stopped by max_rounds leaves vertices piled and stamps set; the
+    next solve and a timing collection must start clean."""
+    src, dst, w = S.rmat(13, 16, seed=5, threads=8)
+    V = 1 << 13
+    ref = oracle.dijkstra(src, dst, w, V, 3)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("delta", 0.01)
+        g.set_param("hub_degree", 32)
+        g.set_param("max_rounds", 4)
+        with pytest.raises(S.SSSPError):
+            g.solve(3, "push")
+        g.set_param("max_rounds", 100000)
+        d, _ = g.solve(3, "push")
+        assert np.array_equal(bits(d), bits(ref))
+        feats, times = g.collect_timings(3)
+        assert len(feats) > 0 and np.all(np.isfinite(times[:, :8]))
+        d, _ = g.solve(3, "heuristic")
+        assert np.array_equal(bits(d), bits(ref))
