@@ -139,6 +139,32 @@ __device__ __forceinline__ void lower_dist(uint32_t *dist, int32_t v, uint32_t c
     __hip_atomic_fetch_min(&dist[v], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Per-workgroup filter of lowering atomics. Device-scope atomics on the
+// distance array execute past the L2, whose copy of the line keeps the value
+// read before them, so every edge of a round into a much-reached vertex would
+// send its own atomic. The filter keeps, per hashed vertex slot, the smallest
+// value this workgroup has sent in this kernel: a candidate not below it
+// cannot lower the distance (already, or once the earlier atomic lands, at
+// most that value) and is dropped; the lane that sent the earlier value
+// appended the vertex. Entries are (vertex << 32 | value), one 64-bit LDS word.
+constexpr int kFiltBits = 10;
+constexpr int kFiltSize = 1 << kFiltBits;
+
+__device__ __forceinline__ void filt_init(uint64_t *f)
+{
+    for (int i = threadIdx.x; i < kFiltSize; i += blockDim.x) f[i] = ~0ull;
+}
+
+// true: send the atomic (the slot now records cand for v)
+__device__ __forceinline__ bool filt_send(uint64_t *f, int32_t v, uint32_t cand)
+{
+    const uint32_t h = ((uint32_t)v * 0x9E3779B1u) >> (32 - kFiltBits);
+    const uint64_t e = f[h];
+    if ((uint32_t)(e >> 32) == (uint32_t)v && (uint32_t)e <= cand) return false;
+    f[h] = ((uint64_t)(uint32_t)v << 32) | cand;
+    return true;
+}
+
 template <class WT, int DIR, int SRC, int PRED, int K>
 __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
 {
@@ -149,11 +175,13 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
     __shared__ uint32_t s_d[TE + 1];
     __shared__ int64_t s_start[SRC == SRC_SPARSE ? TE + 1 : 1];
     __shared__ int32_t s_stage[kBlock / 64][kStageCap];
+    __shared__ uint64_t s_filt[DIR == DIR_PUSH ? kFiltSize : 1];
 
     const int tid = threadIdx.x, lane = __lane_id();
     int32_t *wbuf = s_stage[tid >> 6];
     int wcnt = 0;
     RoundAcc acc;
+    if constexpr (DIR == DIR_PUSH) filt_init(s_filt);  // first use is after a barrier
     if (a.dev_n) {
         a.n = (int32_t)*a.dev_n;
         a.T = *a.dev_T;
@@ -280,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                         // certificate: tight edge into a finite vertex, first claimer wins
                         if (dv != INF && cand == dv && atomicCAS(&a.parent[v], -1, s_v[jj]) == -1)
                             low = true;
-                    } else if (cand < dv) {
+                    } else if (cand < dv && filt_send(s_filt, v, cand)) {
                         lower_dist(a.dist, v, cand);
                         low = true;
                         acc.minw = min(acc.minw, cand);
@@ -316,9 +344,12 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
     constexpr uint32_t INF = WT::INF;
     __shared__ int32_t s_stage[kBlock / 64][kStageCap];
+    __shared__ uint64_t s_filt[kFiltSize];
     int32_t *wbuf = s_stage[threadIdx.x >> 6];
     int wcnt = 0;
     RoundAcc acc;
+    filt_init(s_filt);
+    __syncthreads();
     // XCD-partitioned list (see k_el_pass): group blockIdx % 8, padded to
     // whole 16-byte vectors
     const int64_t x = blockIdx.x & 7, nbx = gridDim.x >> 3, bi = blockIdx.x >> 3;
@@ -372,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
                 v = et[i];
                 const uint32_t cand = WT::add(du[i], ew[i]);
                 acc.relaxed++;
-                if (cand < dv[i]) {
+                if (cand < dv[i] && filt_send(s_filt, v, cand)) {
                     lower_dist(dist, v, cand);
                     low = true;
                     acc.minw = min(acc.minw, cand);
