@@ -12,14 +12,26 @@ Every source is solved with the learned, heuristic and plain schedules outside
 the timed region; the three distance arrays must be bit-identical and the
 learned one must pass the GPU certificate.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+HBM traffic of the all-edge pass (roofline.traffic): before this process
+touches the GPU it runs itself twice under `rocprofv3 --pmc` (FETCH_SIZE, then
+WRITE_SIZE) in --pmc-child mode, which copies 1 GiB device to device and runs
+one plain solve; each counter is scaled by its value for the copy.
+--traffic off skips this (traffic = null).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--traffic auto|off]
 """
 import argparse
+import csv
 import ctypes as C
+import glob
 import json
 import os
 import platform
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -94,6 +106,91 @@ def cpu_baseline(src, dst, w, V, source, plain_passes, max_passes, threads):
     }
 
 
+COPY_BYTES = 1 << 30
+PASS_KERNEL = "k_el_pass"
+
+
+def pmc_child(args):
+    """Profiled child: a 1 GiB device-to-device copy (calibration), then one
+    plain solve of the bench graph from the first bench source; prints the
+    pass count."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(COPY_BYTES)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if (hip.hipMemset(a, 1, C.c_size_t(COPY_BYTES)) != 0 or hip.hipDeviceSynchronize() != 0
+            or hip.hipMemcpy(b, a, C.c_size_t(COPY_BYTES), 3) != 0
+            or hip.hipDeviceSynchronize() != 0):
+        raise RuntimeError("calibration copy failed")
+    hip.hipFree(a)
+    hip.hipFree(b)
+    g = S.DeviceGraph.rmat(args.scale, args.edge_factor, seed=args.seed, threads=64, csc=False)
+    source = bench_sources(g, args)[0]
+    _, st = g.solve(int(source), "plain", download=False)
+    print(json.dumps({"passes": int(st["iterations"]), "source": int(source)}), flush=True)
+
+
+def bench_sources(g, args):
+    """The bench sources: pick_sources depends only on which vertices have
+    out-edges, read here from the device CSR."""
+    deg = np.diff(g.layout(csc=False)["row_ptr"])
+    return S.pick_sources(np.nonzero(deg)[0].astype(np.int32), g.V, args.seed, args.sources)
+
+
+def _pmc_run(counter, args, workdir):
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    out = os.path.join(workdir, counter)
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, os.path.abspath(__file__), "--pmc-child", "--scale", str(args.scale),
+           "--edge-factor", str(args.edge_factor), "--seed", str(args.seed), "--sources",
+           str(args.sources)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                         text=True, start_new_session=True)
+    try:
+        so, se = p.communicate(timeout=240)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+    if p.returncode != 0:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {p.returncode}: {se[-400:]}")
+    passes = None
+    for line in so.splitlines():
+        if line.startswith("{"):
+            passes = json.loads(line)["passes"]
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    if passes is None or not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: no output")
+    copy_kib, pass_kib = [], []
+    with open(files[0]) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            name, v = r["Kernel_Name"], float(r["Counter_Value"])
+            if "copyBuffer" in name:
+                copy_kib.append(v)
+            elif PASS_KERNEL in name:
+                pass_kib.append(v)
+    if not copy_kib or len(pass_kib) < passes:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: dispatches missing")
+    # the copy's counter in KiB against the bytes it moves; the solve's first
+    # `passes` pass dispatches are the real passes (later queued ones return)
+    factor = COPY_BYTES / (max(copy_kib) * 1024.0)
+    per_pass = float(np.mean(pass_kib[:passes])) * 1024.0 * factor
+    return per_pass, factor, passes
+
+
+def measure_traffic(args):
+    """HBM bytes per all-edge pass (FETCH + WRITE, copy-calibrated)."""
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        fetch, ffac, passes = _pmc_run("FETCH_SIZE", args, td)
+        write, wfac, _ = _pmc_run("WRITE_SIZE", args, td)
+    return {"bytes_per_pass": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_factor": ffac, "write_factor": wfac, "passes": passes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,11 +203,27 @@ def main():
     ap.add_argument("--cpu-passes", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--traffic", choices=("auto", "off"), default="auto")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched "
               f"sources of this path would shard across GPUs, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    # host side only so far: the profiled children run before this process
+    # initialises the GPU
+    traffic = None
+    if args.traffic == "auto":
+        t0 = time.perf_counter()
+        try:
+            traffic = measure_traffic(args)
+            log(f"traffic: {json.dumps(traffic)} ({time.perf_counter() - t0:.0f} s)")
+        except Exception as e:  # the bench line stands without it
+            log(f"traffic measurement skipped: {e}")
 
     info = S.device_info(0)
     log("device", info)
@@ -121,13 +234,7 @@ def main():
     g, (h_src, h_dst, h_w) = res if want_host else (res, (None, None, None))
     V, E = g.V, g.E
     log(f"graph: scale {args.scale} V={V} E={E} built on device in {time.perf_counter()-t0:.1f} s")
-    if h_src is not None:
-        sources = S.pick_sources(h_src, V, args.seed, args.sources)
-    else:
-        L = g.layout(csc=False)
-        deg = np.diff(L["row_ptr"])
-        sources = S.pick_sources(np.repeat(np.arange(V, dtype=np.int32), deg), V, args.seed,
-                                 args.sources)
+    sources = bench_sources(g, args)
     log("sources", sources.tolist())
 
     mlp = None
@@ -180,7 +287,9 @@ def main():
     roofline = {"bound": "hbm",
                 "kernel": "k_el_pass (all-edge pass, XCD-partitioned src-sorted edge list)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic["bytes_per_pass"]) if traffic else None,
+                "traffic_detail": traffic,
                 "algorithmic_bytes_per_pass": algo_bytes, "avg_pass_ms": round(pass_ms, 4),
                 "passes": int(len(tr["ms"]))}
 
