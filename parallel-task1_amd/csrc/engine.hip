@@ -1245,95 +1245,111 @@ void solve_plain(sssp_graph *g, sssp_stats *st)
     st->edges_relaxed = (int64_t)(done_at + 1) * E;
 }
 
+// One round r: round-begin kernel, decision, host read-back of the frontier
+// counters, then the round itself. Returns false, having run nothing, when
+// the solve is over (no frontier, no pile, fixpoint check done or off).
+// force_mode >= 0 replaces the decision (not the fixpoint check).
 template <class WT>
-void solve_rounds(sssp_graph *g, int32_t source, int sched, int32_t tick0, sssp_stats *st)
+bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *st,
+                bool &verified, int64_t &relaxed, int force_mode = -1, int force_chunk = 0)
 {
     hipStream_t s = g->stream;
     const bool learned = (sched == SSSP_SCHED_LEARNED);
     const int ns = g->n_sample;
+    k_round_begin<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v, ns,
+                                                      g->dist, g->has_csc ? g->col_ptr : nullptr);
+    SSSP_LAUNCH_CHECK("k_round_begin");
+    DecideArgs da{};
+    da.r = r;
+    da.sched = sched;
+    da.V = g->V;
+    da.E = (double)g->E;
+    da.has_csc = g->has_csc ? 1 : 0;
+    da.push_edge_frac = (float)g->push_edge_frac;
+    da.pull_frac = (float)g->pull_frac;
+    da.push_chunk = g->push_chunk;
+    da.all_chunk = g->all_chunk;
+    da.pull_chunk = g->pull_chunk;
+    da.mlp = (learned && g->mlp) ? mlp_device_params(g->mlp) : nullptr;
+    da.pile = g->pile_cur;
+    k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
+    SSSP_LAUNCH_CHECK("k_decide");
+    read_slot(g, r);
+    const RoundSlot cur = g->ctrl_host->slot[r & 1];
+    const int32_t piled = g->ctrl_host->pile_n[g->pile_cur];
+    const int64_t piled_edges = g->ctrl_host->pile_edges[g->pile_cur];
+    relaxed += cur.relaxed;
+    bool verify = false;
+    if (cur.qlen == 0 && piled == 0) {
+        if (!g->verify || verified) return false;
+        verify = true;
+    }
+    int mode, kidx;
+    switch (sched) {
+    case SSSP_SCHED_PUSH: mode = SSSP_MODE_PUSH; kidx = g->push_chunk; break;
+    case SSSP_SCHED_ALL: mode = SSSP_MODE_ALL; kidx = g->all_chunk; break;
+    case SSSP_SCHED_PULL:
+        mode = g->has_csc ? SSSP_MODE_PULL : SSSP_MODE_ALL;
+        kidx = g->pull_chunk;
+        break;
+    default:
+        mode = g->ctrl_host->dec_mode;
+        kidx = g->ctrl_host->dec_chunk;
+        if (g->ctrl_host->dec_mlp) st->mlp_rounds++;
+        break;
+    }
+    if (force_mode >= 0) {
+        mode = force_mode;
+        kidx = force_chunk;
+    }
+    if (mode == SSSP_MODE_PULL && !g->has_csc) mode = SSSP_MODE_ALL;
+    if (kidx < 0 || kidx >= kChunkCount) kidx = 1;
+    if (!verify && cur.qlen == 0 && mode != SSSP_MODE_PUSH && force_mode < 0) {
+        // only the pile is pending: a push round drains it
+        mode = SSSP_MODE_PUSH;
+        kidx = g->push_chunk;
+    }
+    if (verify) {
+        mode = SSSP_MODE_ALL;
+        kidx = g->all_chunk;
+        st->verify_passes++;
+        verified = true;
+    } else {
+        verified = false;
+    }
+    if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
+    if (run_round<WT>(g, mode, kidx, r, tick0, cur, verify)) g->pile_cur ^= 1;
+    float ms = 0.f;
+    if (g->trace_events) {
+        SSSP_HIP(hipEventRecord(g->ev_r1, s));
+        SSSP_HIP(hipEventSynchronize(g->ev_r1));
+        SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
+    }
+    TraceRow tr{};
+    tr.mode = verify ? 3 : mode;
+    tr.chunk = kidx;
+    tr.frontier = (int64_t)cur.qlen + piled;
+    tr.frontier_edges = verify ? g->E : cur.qedges + piled_edges;
+    tr.ms = ms;
+    tr.unsettled = cur.sample_n > 0 ? (float)cur.sample_unsettled / cur.sample_n : 0.f;
+    g->trace.push_back(tr);
+    st->iterations++;
+    if (mode == SSSP_MODE_PUSH) st->n_push++;
+    else if (mode == SSSP_MODE_ALL) st->n_all++;
+    else st->n_pull++;
+    return true;
+}
+
+template <class WT>
+void solve_rounds(sssp_graph *g, int sched, int32_t tick0, sssp_stats *st)
+{
     bool verified = false;
     int64_t relaxed = 0;
     g->pile_cur = 0;
     for (int32_t r = 0;; r++) {
         if (r >= g->max_rounds)
             throw Error{SSSP_ERR_NOT_CONVERGED, "solve did not converge within max_rounds"};
-        k_round_begin<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v, ns,
-                                                          g->dist, g->has_csc ? g->col_ptr : nullptr);
-        SSSP_LAUNCH_CHECK("k_round_begin");
-        DecideArgs da{};
-        da.r = r;
-        da.sched = sched;
-        da.V = g->V;
-        da.E = (double)g->E;
-        da.has_csc = g->has_csc ? 1 : 0;
-        da.push_edge_frac = (float)g->push_edge_frac;
-        da.pull_frac = (float)g->pull_frac;
-        da.push_chunk = g->push_chunk;
-        da.all_chunk = g->all_chunk;
-        da.pull_chunk = g->pull_chunk;
-        da.mlp = (learned && g->mlp) ? mlp_device_params(g->mlp) : nullptr;
-        da.pile = g->pile_cur;
-        k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
-        SSSP_LAUNCH_CHECK("k_decide");
-        read_slot(g, r);
-        const RoundSlot cur = g->ctrl_host->slot[r & 1];
-        const int32_t piled = g->ctrl_host->pile_n[g->pile_cur];
-        const int64_t piled_edges = g->ctrl_host->pile_edges[g->pile_cur];
-        relaxed += cur.relaxed;
-        bool verify = false;
-        if (cur.qlen == 0 && piled == 0) {
-            if (!g->verify || verified) break;
-            verify = true;
-        }
-        int mode, kidx;
-        switch (sched) {
-        case SSSP_SCHED_PUSH: mode = SSSP_MODE_PUSH; kidx = g->push_chunk; break;
-        case SSSP_SCHED_ALL: mode = SSSP_MODE_ALL; kidx = g->all_chunk; break;
-        case SSSP_SCHED_PULL:
-            mode = g->has_csc ? SSSP_MODE_PULL : SSSP_MODE_ALL;
-            kidx = g->pull_chunk;
-            break;
-        default:
-            mode = g->ctrl_host->dec_mode;
-            kidx = g->ctrl_host->dec_chunk;
-            if (g->ctrl_host->dec_mlp) st->mlp_rounds++;
-            break;
-        }
-        if (mode == SSSP_MODE_PULL && !g->has_csc) mode = SSSP_MODE_ALL;
-        if (kidx < 0 || kidx >= kChunkCount) kidx = 1;
-        if (!verify && cur.qlen == 0) {
-            // only the pile is pending: a push round drains it
-            mode = SSSP_MODE_PUSH;
-            kidx = g->push_chunk;
-        }
-        if (verify) {
-            mode = SSSP_MODE_ALL;
-            kidx = g->all_chunk;
-            st->verify_passes++;
-            verified = true;
-        } else {
-            verified = false;
-        }
-        if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
-        if (run_round<WT>(g, mode, kidx, r, tick0, cur, verify)) g->pile_cur ^= 1;
-        float ms = 0.f;
-        if (g->trace_events) {
-            SSSP_HIP(hipEventRecord(g->ev_r1, s));
-            SSSP_HIP(hipEventSynchronize(g->ev_r1));
-            SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
-        }
-        TraceRow tr{};
-        tr.mode = verify ? 3 : mode;
-        tr.chunk = kidx;
-        tr.frontier = (int64_t)cur.qlen + piled;
-        tr.frontier_edges = verify ? g->E : cur.qedges + piled_edges;
-        tr.ms = ms;
-        tr.unsettled = cur.sample_n > 0 ? (float)cur.sample_unsettled / cur.sample_n : 0.f;
-        g->trace.push_back(tr);
-        st->iterations++;
-        if (mode == SSSP_MODE_PUSH) st->n_push++;
-        else if (mode == SSSP_MODE_ALL) st->n_all++;
-        else st->n_pull++;
+        if (!step_round<WT>(g, r, sched, tick0, st, verified, relaxed)) break;
     }
     st->edges_relaxed = relaxed;
 }
@@ -1383,7 +1399,7 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
         }
         g->pile_dirty = true;
         try {
-            solve_rounds<WT>(g, source, sched, tick0, st);
+            solve_rounds<WT>(g, sched, tick0, st);
         } catch (...) {
             g->tick = tick0 + g->max_rounds + 2;
             throw;
@@ -1449,32 +1465,20 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
     // with a parent + the source
 }
 
-// Restores the control block's round state before a timed try: the slot the
-// round fills, the pile it reads (host values) and the counters it fills.
-__global__ void k_restore_round(Ctrl *c, int32_t r, int p, int32_t qlen, int32_t pn, uint32_t pmin,
-                                int64_t pedges)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    zero_slot(&c->slot[(r + 1) & 1], 0xFFFFFFFFu);
-    c->pile_n[p] = pn;
-    c->pile_min[p] = pmin;
-    c->pile_edges[p] = pedges;
-    c->pile_n[p ^ 1] = 0;
-    c->pile_min[p ^ 1] = 0xFFFFFFFFu;
-    c->pile_edges[p ^ 1] = 0;
-    c->lb_n = qlen;
-}
-
-// Timing collection: before every round, each mode x chunk bucket is run from
-// the same saved state (distances, stamps and pile states restored, control
-// block restored) and timed with device events; then the fastest is run for
-// real.
+// Timing collection with rollouts. At every round r of a solve the state is
+// saved (distances, stamps, pile states, the round's queue and pile, the
+// control block) and each mode x chunk bucket c is tried: round r runs c and
+// the rounds after it follow the hand-written rule to the end of the solve;
+// times[r][c] is the device time of that whole remainder (host round trips
+// included, as in a real solve). The solve then advances by the best c. The
+// labels are thus one step of policy improvement over the rule: a round that
+// costs more now but leaves less work scores by the total.
 template <class WT>
 void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
                int32_t *rows)
 {
     hipStream_t s = g->stream;
-    if (g->tick > (1 << 30) - g->max_rounds - 4) {
+    if (g->tick > (1 << 30) - 2 * g->max_rounds - 4) {
         reset_stamps(g);
         g->tick = 1;
     }
@@ -1484,77 +1488,99 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
     }
     const int32_t tick0 = g->tick;
     const size_t V = (size_t)g->V;
-    uint32_t *sd = dmalloc<uint32_t>(V);
-    int32_t *sst = nullptr, *sfa = nullptr;
-    struct Free {
-        void *a = nullptr, *b = nullptr, *c = nullptr;
-        ~Free()
+    struct Saved {
+        void *p[6] = {};
+        ~Saved()
         {
-            dfree(a);
-            dfree(b);
-            dfree(c);
+            for (void *x : p) dfree(x);
         }
-    } fr;
-    fr.a = sd;
-    sst = dmalloc<int32_t>(V);
-    fr.b = sst;
-    sfa = dmalloc<int32_t>(V);
-    fr.c = sfa;
+    } sv;
+    uint32_t *sd = dmalloc<uint32_t>(V);
+    sv.p[0] = sd;
+    int32_t *sst = dmalloc<int32_t>(V);
+    sv.p[1] = sst;
+    int32_t *sfa = dmalloc<int32_t>(V);
+    sv.p[2] = sfa;
+    int32_t *sq = dmalloc<int32_t>(V + 1);
+    sv.p[3] = sq;
+    int32_t *sp = dmalloc<int32_t>(V);
+    sv.p[4] = sp;
+    Ctrl *sc = dmalloc<Ctrl>(1);
+    sv.p[5] = sc;
+    const int trace_was = g->trace_events;
+    g->trace_events = 0;
+    g->trace.clear();
     g->pile_dirty = true;
     g->pile_cur = 0;
     k_fill<uint32_t><<<grid_for(g->V), kBlock, 0, s>>>(g->dist, g->V, WT::INF);
     k_init_solve<<<1, 64, 0, s>>>(g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, source, tick0);
     SSSP_LAUNCH_CHECK("collect init");
     const int nt = kModes * kChunkCount;
-    int32_t row = 0, r = 0;
+    int32_t row = 0, r = 0, r_hi = 0;
+    bool verified = false;
+    int64_t relaxed = 0;
+    sssp_stats st{};
+    auto copy = [&](void *dst, const void *src, size_t bytes) {
+        SSSP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+    };
     for (;; r++) {
         if (r >= g->max_rounds) {
-            g->tick = tick0 + g->max_rounds + 2;
+            g->tick = tick0 + 2 * g->max_rounds + 2;
+            g->trace_events = trace_was;
             throw Error{SSSP_ERR_NOT_CONVERGED, "collect: too many rounds"};
         }
         const int p = g->pile_cur;
-        k_round_begin<<<grid_for(g->n_sample, 64), kBlock, 0, s>>>(
-            g->ctrl, r, p, g->sample_v, g->n_sample, g->dist, g->has_csc ? g->col_ptr : nullptr);
-        DecideArgs da{};
-        da.r = r;
-        da.sched = SSSP_SCHED_HEURISTIC;
-        da.V = g->V;
-        da.E = (double)g->E;
-        da.has_csc = g->has_csc;
-        da.push_edge_frac = (float)g->push_edge_frac;
-        da.pull_frac = (float)g->pull_frac;
-        da.pile = p;
-        k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
-        SSSP_LAUNCH_CHECK("collect decide");
-        read_slot(g, r);
-        const RoundSlot cur = g->ctrl_host->slot[r & 1];
-        const int32_t pn = g->ctrl_host->pile_n[p];
-        const uint32_t pmin = g->ctrl_host->pile_min[p];
-        const int64_t pe = g->ctrl_host->pile_edges[p];
-        if (cur.qlen == 0 && pn == 0) break;
-        SSSP_HIP(hipMemcpyAsync(sd, g->dist, 4 * V, hipMemcpyDeviceToDevice, s));
-        SSSP_HIP(hipMemcpyAsync(sst, g->stamp, 4 * V, hipMemcpyDeviceToDevice, s));
-        SSSP_HIP(hipMemcpyAsync(sfa, g->farst, 4 * V, hipMemcpyDeviceToDevice, s));
+        copy(sd, g->dist, 4 * V);
+        copy(sst, g->stamp, 4 * V);
+        copy(sfa, g->farst, 4 * V);
+        copy(sq, g->queue[r & 1], 4 * (V + 1));
+        copy(sp, g->pile[p], 4 * V);
+        copy(sc, g->ctrl, sizeof(Ctrl));
         auto restore = [&] {
-            SSSP_HIP(hipMemcpyAsync(g->dist, sd, 4 * V, hipMemcpyDeviceToDevice, s));
-            SSSP_HIP(hipMemcpyAsync(g->stamp, sst, 4 * V, hipMemcpyDeviceToDevice, s));
-            SSSP_HIP(hipMemcpyAsync(g->farst, sfa, 4 * V, hipMemcpyDeviceToDevice, s));
-            k_restore_round<<<1, 64, 0, s>>>(g->ctrl, r, p, cur.qlen, pn, pmin, pe);
-            SSSP_LAUNCH_CHECK("k_restore_round");
+            copy(g->dist, sd, 4 * V);
+            copy(g->stamp, sst, 4 * V);
+            copy(g->farst, sfa, 4 * V);
+            copy(g->queue[r & 1], sq, 4 * (V + 1));
+            copy(g->pile[p], sp, 4 * V);
+            copy(g->ctrl, sc, sizeof(Ctrl));
+            g->pile_cur = p;
         };
         float t[kModes * kChunkCount];
+        float f[kIn];
         float best = 3.0e38f;
-        int bm = 0, bc = 0;
-        for (int mode = 0; mode < kModes; mode++) {
+        int bm = -1, bc = 0;
+        bool over = false, verify_round = false;
+        for (int mode = 0; mode < kModes && !over; mode++) {
             for (int c = 0; c < kChunkCount; c++) {
                 float ms = -1.f;
                 if (!(mode == SSSP_MODE_PULL && !g->has_csc)) {
                     restore();
-                    SSSP_HIP(hipEventRecord(g->ev_r0, s));
-                    (void)run_round<WT>(g, mode, c, r, tick0, cur, false);
-                    SSSP_HIP(hipEventRecord(g->ev_r1, s));
-                    SSSP_HIP(hipEventSynchronize(g->ev_r1));
-                    SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
+                    SSSP_HIP(hipEventRecord(g->ev_a, s));
+                    bool ver = verified;  // the rollout continues this solve
+                    int64_t rel = 0;
+                    sssp_stats tmp{};
+                    if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &tmp, ver, rel, mode, c)) {
+                        over = true;  // nothing pending at round r: the solve is complete
+                        break;
+                    }
+                    if (bm < 0) {
+                        for (int i = 0; i < kIn; i++) f[i] = g->ctrl_host->feat[i];
+                        // a fixpoint check runs whatever the choice: no label
+                        verify_round = g->trace.back().mode == 3;
+                    }
+                    int32_t r2 = r + 1;
+                    for (;; r2++) {
+                        if (r2 >= 2 * g->max_rounds) {
+                            g->tick = tick0 + 2 * g->max_rounds + 2;
+                            g->trace_events = trace_was;
+                            throw Error{SSSP_ERR_NOT_CONVERGED, "collect: rollout did not converge"};
+                        }
+                        if (!step_round<WT>(g, r2, SSSP_SCHED_HEURISTIC, tick0, &tmp, ver, rel)) break;
+                    }
+                    r_hi = r2 > r_hi ? r2 : r_hi;
+                    SSSP_HIP(hipEventRecord(g->ev_b, s));
+                    SSSP_HIP(hipEventSynchronize(g->ev_b));
+                    SSSP_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
                     if (ms < best) {
                         best = ms;
                         bm = mode;
@@ -1565,16 +1591,20 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
             }
         }
         restore();
-        if (run_round<WT>(g, bm, bc, r, tick0, cur, false)) g->pile_cur ^= 1;
-        if (row < cap) {
-            for (int i = 0; i < kIn; i++) feats[(size_t)row * kIn + i] = g->ctrl_host->feat[i];
+        if (over) break;
+        if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &st, verified, relaxed, bm, bc))
+            break;
+        if (!verify_round && row < cap) {
+            for (int i = 0; i < kIn; i++) feats[(size_t)row * kIn + i] = f[i];
             for (int i = 0; i < nt; i++) times[(size_t)row * nt + i] = t[i];
             row++;
         }
     }
     SSSP_HIP(hipStreamSynchronize(s));
+    g->trace.clear();
+    g->trace_events = trace_was;
     g->pile_dirty = false;
-    g->tick = tick0 + r + 2;
+    g->tick = tick0 + (r_hi > r ? r_hi : r) + 2;
     *rows = row;
 }
 

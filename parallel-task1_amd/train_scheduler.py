@@ -1,17 +1,20 @@
 """Trains the learned round scheduler on timings the solver itself measures.
 
-For directed R-MAT graphs of scales 18..22 (edge factor 16, a/b/c/d =
-57/19/19/5, generated on the device) and several sources each,
-sssp_collect_timings runs every relaxation round three ways (push, all-edge,
-pull) at each of the four chunk widths from the same saved state, times them
-with device events, and continues with the fastest. Each round becomes one
-training row: the 5 round features -> the fastest mode and, for that mode, the
-fastest chunk bucket. The network (5 -> 64 -> 64 -> 3 + 4) is trained on the
-GPU by the library's own forward/backward/Adam kernels, and the weights are
-written to models/scheduler_mlp.f32 (raw float32, the library's parameter
-order) with a JSON sidecar describing the run.
+For directed R-MAT graphs (edge factor 16, a/b/c/d = 57/19/19/5, generated on
+the device) and several sources each, sssp_collect_timings saves the solve
+state before every round and tries each of the 12 choices (push, all-edge,
+pull x four chunk widths): the round runs that choice and the rest of the
+solve follows the hand-written rule; the device time of the whole remainder is
+the choice's cost (a rollout). The solve then advances by the best choice.
+Each round becomes one training row: the 5 round features -> the mode with the
+lowest remainder time and, for that mode, the best chunk bucket. A network
+trained on these labels is one step of policy improvement over the rule. It
+(5 -> 64 -> 64 -> 3 + 4) is trained on the GPU by the library's own
+forward/backward/Adam kernels, and the weights are written to
+models/scheduler_mlp.f32 (raw float32, the library's parameter order) with a
+JSON sidecar describing the run.
 
-    python parallel-task1_amd/train_scheduler.py [--scales 18 19 20 21 22]
+    python parallel-task1_amd/train_scheduler.py [--scales 19 20 21 22 23 24]
 """
 import argparse
 import json
@@ -68,8 +71,8 @@ def regret(t, pm, pc):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scales", type=int, nargs="+", default=[18, 19, 20, 21, 22])
-    ap.add_argument("--sources", type=int, default=6)
+    ap.add_argument("--scales", type=int, nargs="+", default=[19, 20, 21, 22, 23, 24])
+    ap.add_argument("--sources", type=int, default=4)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--epochs", type=int, default=400)
