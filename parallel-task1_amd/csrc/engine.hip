@@ -378,11 +378,15 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
         for (int i = 0; i < 4; i++) {
             du[i] = INF;
             if (val[i]) {
-                const uint32_t d = dist[es[i]];
-                bool act = d != INF;
-                if constexpr (PRED == PRED_STAMP) act = act && stamp[es[i]] >= tick_prev;
-                du[i] = act ? d : INF;
-                any |= act;
+                if (i > 0 && es[i] == es[i - 1]) {
+                    du[i] = du[i - 1];  // same source (src-sorted list): same predicate
+                } else {
+                    const uint32_t d = dist[es[i]];
+                    bool act = d != INF;
+                    if constexpr (PRED == PRED_STAMP) act = act && stamp[es[i]] >= tick_prev;
+                    du[i] = act ? d : INF;
+                }
+                any |= du[i] != INF;
             }
         }
         uint32_t dv[4];
@@ -869,6 +873,12 @@ __global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ 
     if (pass > 0 && __hip_atomic_load(&changed[(pass - 1) & 63], __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT) == 0)
         return;
+    constexpr bool kUseFilt = sizeof(DT) == 4 && VEC == 4;
+    __shared__ uint64_t s_filt[kUseFilt ? kFiltSize : 1];
+    if constexpr (kUseFilt) {
+        filt_init(s_filt);
+        __syncthreads();
+    }
     bool low = false;
     // both distances of an edge are loaded before either is used, so the
     // random d[dst] gather does not wait for the d[src] test
@@ -876,6 +886,8 @@ __global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ 
         if (du == WT::INF) return;
         const DT cand = WT::add(du, ww);
         if (cand < dv) {
+            if constexpr (kUseFilt)
+                if (!filt_send(s_filt, t, (uint32_t)cand)) return;
             // dv was read in this pass: the distance drops in this pass
             // whoever wins, so the result of the min is not needed
             __hip_atomic_fetch_min(&dist[t], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -907,7 +919,12 @@ __global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ 
             const v4i s = __builtin_nontemporal_load(&s4[g]);
             const v4i t = __builtin_nontemporal_load(&t4[g]);
             const v4u ww = __builtin_nontemporal_load(&w4[g]);
-            const DT u0 = dist[s.x], u1 = dist[s.y], u2 = dist[s.z], u3 = dist[s.w];
+            // the list is src-sorted: a lane's next edge usually has the
+            // same source, whose distance is then not loaded again
+            const DT u0 = dist[s.x];
+            const DT u1 = s.y == s.x ? u0 : dist[s.y];
+            const DT u2 = s.z == s.y ? u1 : dist[s.z];
+            const DT u3 = s.w == s.z ? u2 : dist[s.w];
             const DT v0 = dist[t.x], v1 = dist[t.y], v2 = dist[t.z], v3 = dist[t.w];
             relax(u0, v0, t.x, (DT)ww.x);
             relax(u1, v1, t.y, (DT)ww.y);
