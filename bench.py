@@ -182,6 +182,35 @@ def _pmc_run(counter, args, workdir):
     return per_pass, factor, passes
 
 
+def copy_bandwidth(reps=5):
+    """Achievable HBM bandwidth on this GPU: a 1 GiB device-to-device copy
+    (reads and writes 1 GiB), best of `reps`, timed with HIP events."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    ok = (hip.hipMalloc(C.byref(a), C.c_size_t(COPY_BYTES)) == 0
+          and hip.hipMalloc(C.byref(b), C.c_size_t(COPY_BYTES)) == 0
+          and hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0)
+    best = None
+    if ok and hip.hipMemset(a, 1, C.c_size_t(COPY_BYTES)) == 0:
+        for _ in range(reps):
+            ms = C.c_float()
+            if (hip.hipEventRecord(e0, None) != 0
+                    or hip.hipMemcpyAsync(b, a, C.c_size_t(COPY_BYTES), 3, None) != 0
+                    or hip.hipEventRecord(e1, None) != 0 or hip.hipEventSynchronize(e1) != 0
+                    or hip.hipEventElapsedTime(C.byref(ms), e0, e1) != 0):
+                best = None
+                break
+            best = ms.value if best is None else min(best, ms.value)
+    for p in (a, b):
+        if p.value:
+            hip.hipFree(p)
+    for e in (e0, e1):
+        if e.value:
+            hip.hipEventDestroy(e)
+    return None if not best else 2 * COPY_BYTES / (best * 1e-3) / 1e9
+
+
 def measure_traffic(args):
     """HBM bytes per all-edge pass (FETCH + WRITE, copy-calibrated)."""
     with tempfile.TemporaryDirectory(dir="/tmp") as td:
@@ -272,6 +301,9 @@ def main():
             "pull": float(np.mean([s["n_pull"] for s in stats])),
             "mlp_rounds": float(np.mean([s["mlp_rounds"] for s in stats])),
             "edges_relaxed_per_E": float(np.mean([s["edges_relaxed"] for s in stats])) / E,
+            # per-source device GTEPS (sample/sssp.cpp:148-151 averages per source)
+            "gteps_source_mean": float(np.mean([E / (t * 1e-3) / 1e9 for t in tdev])),
+            "gteps_source_hmean": float(len(tdev) / np.sum([(t * 1e-3) * 1e9 / E for t in tdev])),
         }
         log(sched, json.dumps(results[sched]))
 
@@ -284,6 +316,8 @@ def main():
     pass_ms = float(np.mean(tr["ms"]))
     algo_bytes = 12 * E + 8 * V
     achieved = algo_bytes / (pass_ms * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth()
+    log(f"device-to-device copy: {copy_gbs} GB/s")
     roofline = {"bound": "hbm",
                 "kernel": "k_el_pass (all-edge pass, XCD-partitioned src-sorted edge list)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -291,6 +325,7 @@ def main():
                 "traffic": round(traffic["bytes_per_pass"]) if traffic else None,
                 "traffic_detail": traffic,
                 "algorithmic_bytes_per_pass": algo_bytes, "avg_pass_ms": round(pass_ms, 4),
+                "achievable_copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
                 "passes": int(len(tr["ms"]))}
 
     cpu = None

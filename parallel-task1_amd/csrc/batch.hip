@@ -361,7 +361,9 @@ Config default_config(sssp_graph *g, int32_t n)
 {
     if (g->batch_policy)
         return Config{g->batch_policy, g->batch_bs, g->batch_wgpcu, g->push_edge_frac, g->push_chunk};
-    if (n >= g->cu_count / 4) return Config{1, 256, 2, 0.05, 1};
+    // untuned policy 1: 1024-thread workgroups, one per CU (the autotuner's
+    // pick at R-MAT scale 20 x 256 sources and for packs of 8192 graphs)
+    if (n >= g->cu_count / 4) return Config{1, 1024, 1, 0.05, 1};
     return Config{2, 256, 2, g->push_edge_frac, g->push_chunk};
 }
 
@@ -449,7 +451,7 @@ int sssp_solve_batch(sssp_graph *g, const int32_t *sources, int32_t n, int32_t p
             SSSP_REQUIRE(sources[i] >= 0 && sources[i] < g->V, "source vertex out of range");
         SSSP_HIP(hipSetDevice(g->device));
         Config c = default_config(g, n);
-        if (policy != 0 && policy != c.policy) c = (policy == 1) ? Config{1, 256, 2, 0.05, 1}
+        if (policy != 0 && policy != c.policy) c = (policy == 1) ? Config{1, 1024, 1, 0.05, 1}
                                                                  : Config{2, 256, 2, g->push_edge_frac, g->push_chunk};
         const double ms = batch_sources(g, sources, n, c);
         if (dist_out)
@@ -543,7 +545,7 @@ int sssp_pack_solve(sssp_pack *p, const int32_t *sources, int32_t policy, float 
         check_pack_sources(p, sources);
         SSSP_HIP(hipSetDevice(p->g->device));
         Config c{p->policy ? p->policy : 1, p->bs, p->wgpcu, p->frac, p->push_chunk};
-        if (policy == 1) c = Config{1, p->policy == 1 ? p->bs : 256, p->policy == 1 ? p->wgpcu : 2, 0.05, 1};
+        if (policy == 1) c = Config{1, p->policy == 1 ? p->bs : 1024, p->policy == 1 ? p->wgpcu : 1, 0.05, 1};
         if (policy == 2) c = Config{2, 256, 2, p->policy == 2 ? p->frac : 0.05, p->policy == 2 ? p->push_chunk : 1};
         const double ms = pack_run(p, sources, c);
         if (dist_out)

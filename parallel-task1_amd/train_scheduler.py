@@ -1,7 +1,8 @@
 """Trains the learned round scheduler on timings the solver itself measures.
 
 For directed R-MAT graphs (edge factor 16, a/b/c/d = 57/19/19/5, generated on
-the device) and several sources each, sssp_collect_timings saves the solve
+the device) and uniform random graphs (integer weights 1..255), several
+sources each, sssp_collect_timings saves the solve
 state before every round and tries each of the 12 choices (push, all-edge,
 pull x four chunk widths): the round runs that choice and the rest of the
 solve follows the hand-written rule; the device time of the whole remainder is
@@ -53,6 +54,28 @@ def collect(scales, sources_per_scale, edge_factor, seed0):
     return np.concatenate(X), np.concatenate(T), meta
 
 
+def collect_er(logvs, sources_per_graph, degree, seed0):
+    """Uniform random graphs (random_uniform, integer weights 1..255)."""
+    X, T, meta = [], [], []
+    for k, logv in enumerate(logvs):
+        seed = seed0 + k
+        V = 1 << logv
+        src, dst, w = S.er(V, degree, seed=seed, weight_mode=1)
+        g = S.DeviceGraph(src, dst, w, V)
+        srcs = S.pick_sources(src, V, seed, sources_per_graph)
+        del src, dst, w
+        for s in srcs:
+            t0 = time.perf_counter()
+            f, t = g.collect_timings(int(s))
+            X.append(f)
+            T.append(t)
+            meta.append({"er_logv": logv, "seed": seed, "source": int(s), "rounds": len(f),
+                         "seconds": round(time.perf_counter() - t0, 2)})
+            print(f"ER 2^{logv} source {int(s)}: {len(f)} rounds", flush=True)
+        g.close()
+    return np.concatenate(X), np.concatenate(T), meta
+
+
 def labels(T):
     """Fastest mode (over its best chunk) and that mode's fastest chunk."""
     t = T.reshape(len(T), S.MLP_MODES, S.MLP_CHUNKS).astype(np.float64)
@@ -74,6 +97,8 @@ def main():
     ap.add_argument("--scales", type=int, nargs="+", default=[19, 20, 21, 22, 23, 24])
     ap.add_argument("--sources", type=int, default=4)
     ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--er-logv", type=int, nargs="*", default=[20, 21, 22, 23])
+    ap.add_argument("--er-degree", type=int, default=16)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--batch", type=int, default=64)
@@ -82,6 +107,9 @@ def main():
     args = ap.parse_args()
 
     X, T, meta = collect(args.scales, args.sources, args.edge_factor, args.seed)
+    if args.er_logv:
+        Xe, Te, me = collect_er(args.er_logv, args.sources, args.er_degree, args.seed + 100)
+        X, T, meta = np.concatenate([X, Xe]), np.concatenate([T, Te]), meta + me
     ym, yc, t = labels(T)
     n = len(X)
     rng = np.random.default_rng(args.seed)
@@ -99,6 +127,7 @@ def main():
     report = {
         "rows": int(n), "train_rows": int(len(tr)), "val_rows": int(nval),
         "scales": args.scales, "edge_factor": args.edge_factor, "sources_per_scale": args.sources,
+        "er_logv": args.er_logv, "er_degree": args.er_degree,
         "label_mode_counts": np.bincount(ym, minlength=S.MLP_MODES).tolist(),
         "final_train_loss": float(losses[-1]) if len(losses) else None,
         "val_mode_accuracy": float((pm[val] == ym[val]).mean()),

@@ -1,0 +1,137 @@
+"""The workload configurations at full size, on the GPU.
+
+  C2   R-MAT scale 22, edge factor 32 (E = 134 M), base seed 1, 64 virtual
+       generator threads; 16 sources from pick_sources(seed 2)
+  C3   uniform random graph V = 2^24, average degree 16 (E = 268 M), srand(3),
+       integer weights 1 + rand() % 255; source 0
+  C4a  R-MAT scale 20, edge factor 16, seed 4; 256 sources in one batch
+  C4b  8192 R-MAT graphs, scales uniform in {8..16} (numpy default_rng(5)),
+       edge factor 8, graph k seeded 5 + k with one virtual thread; one source
+       per graph: its smallest vertex with out-edges (vertex 0 when it has one)
+
+At these sizes the oracle (Dijkstra, oracle/oracle.c) runs on a sample; every
+result is also checked through properties that hold at any size: the GPU
+certificate (no edge can still relax; a tight parent tree reaches every
+reached vertex), bit equality across schedules, and batched/packed solves
+equal to single solves. The timings printed are the solver's own device times.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import sssp_amd as S
+from conftest import ROOT, bits
+
+WEIGHTS = os.path.join(ROOT, "parallel-task1_amd", "models", "scheduler_mlp.f32")
+
+
+def _with_network(g):
+    if os.path.exists(WEIGHTS):
+        m = S.MLP(0)
+        m.set_params(np.fromfile(WEIGHTS, np.float32))
+        g.set_mlp(m)
+        return m
+    return None
+
+
+def _gteps(E, ms):
+    return E / (ms * 1e-3) / 1e9
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_c2_rmat22_ef32_sixteen_sources(gpu, oracle):
+    g, (src, dst, w) = S.DeviceGraph.rmat(22, 32, seed=1, threads=64, download_edges=True)
+    with g:
+        V, E = g.V, g.E
+        assert E == 134217728
+        m = _with_network(g)
+        sources = S.pick_sources(src, V, 2, 16)
+        t = {"learned": [], "heuristic": [], "plain": []}
+        for s in sources:
+            ref = None
+            for sched in t:
+                d, st = g.solve(int(s), sched)
+                t[sched].append(st["t_solve_ms"])
+                if ref is None:
+                    ref = d
+                    ok, counts, _ = g.certify(int(s), parents=False)
+                    assert ok, (int(s), counts)
+                else:
+                    assert np.array_equal(bits(d), bits(ref)), (int(s), sched)
+        dj = oracle.dijkstra(src, dst, w, V, int(sources[0]))
+        d, _ = g.solve(int(sources[0]), "learned")
+        assert np.array_equal(bits(d), bits(dj))
+        print("C2", {k: round(_gteps(E * len(v), sum(v)), 2) for k, v in t.items()}, "GTEPS")
+        del m
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_c3_er24_u32(gpu, oracle):
+    V = 1 << 24
+    src, dst, w = S.er(V, 16, seed=3, weight_mode=1)
+    assert w.dtype == np.uint32 and len(src) == 268435456
+    assert w.min() >= 1 and w.max() <= 255
+    with S.DeviceGraph(src, dst, w, V) as g:
+        m = _with_network(g)
+        out = {}
+        for sched in ("learned", "heuristic", "plain"):
+            d, st = g.solve(0, sched)
+            out[sched] = (d, st["t_solve_ms"])
+        ok, counts, _ = g.certify(0, parents=False)
+        assert ok, counts
+        for sched in ("heuristic", "plain"):
+            assert np.array_equal(out[sched][0], out["learned"][0]), sched
+        dj = oracle.dijkstra(src, dst, w, V, 0)
+        assert np.array_equal(out["learned"][0], dj)
+        print("C3", {k: round(_gteps(len(src), v[1]), 2) for k, v in out.items()}, "GTEPS")
+        del m
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_c4a_batch_of_256_sources(gpu, oracle):
+    src, dst, w = S.rmat(20, 16, seed=4, threads=64)
+    V = 1 << 20
+    sources = S.pick_sources(src, V, 4, 256)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        D1, st1 = g.solve_batch(sources, policy=1)
+        D2, st2 = g.solve_batch(sources, policy=2)
+        assert np.array_equal(bits(D1), bits(D2))
+        for i in range(256):
+            d, _ = g.solve(int(sources[i]), "heuristic")
+            assert np.array_equal(bits(D1[i]), bits(d)), i
+        for i in (0, 255):
+            dj = oracle.dijkstra(src, dst, w, V, int(sources[i]))
+            assert np.array_equal(bits(D1[i]), bits(dj)), i
+        ok, counts, _ = g.certify(int(sources[255]), parents=False)
+        assert ok, counts
+        print("C4a", {"policy1": round(_gteps(256 * len(src), st1["t_solve_ms"]), 2),
+                      "policy2": round(_gteps(256 * len(src), st2["t_solve_ms"]), 2)}, "GTEPS")
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_c4b_pack_of_8192_graphs(gpu, oracle):
+    n = 8192
+    scales = np.random.default_rng(5).integers(8, 17, n).astype(np.int32)
+    s, d, w, offs = S.rmat_many(scales, 8, seed0=5)
+    vcounts = (1 << scales).astype(np.int32)
+    sources = np.array([int(s[offs[k]:offs[k + 1]].min()) for k in range(n)], np.int32)
+    voff = np.concatenate([[0], np.cumsum(vcounts.astype(np.int64))])
+    pack = S.GraphPack(s, d, w, offs, vcounts)
+    try:
+        D1, st1 = pack.solve(sources, policy=1)
+        D2, st2 = pack.solve(sources, policy=2)
+        assert np.array_equal(bits(D1), bits(D2))
+        for k in np.random.default_rng(6).choice(n, 32, replace=False).tolist() + [0, n - 1]:
+            sl = slice(offs[k], offs[k + 1])
+            dj = oracle.dijkstra(s[sl], d[sl], w[sl], int(vcounts[k]), int(sources[k]))
+            assert np.array_equal(bits(D1[voff[k]:voff[k + 1]]), bits(dj)), k
+        E = len(s)
+        print("C4b", {"policy1": round(_gteps(E, st1["t_solve_ms"]), 2),
+                      "policy2": round(_gteps(E, st2["t_solve_ms"]), 2)}, "GTEPS", E, "edges")
+    finally:
+        pack.close()
