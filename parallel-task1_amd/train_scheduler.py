@@ -1,7 +1,7 @@
 """Trains the learned round scheduler on timings the solver itself measures.
 
-For directed R-MAT graphs (edge factor 16, a/b/c/d = 57/19/19/5, generated on
-the device) and uniform random graphs (integer weights 1..255), several
+For directed R-MAT graphs (edge factors 16 and 32, a/b/c/d = 57/19/19/5,
+generated on the device) and uniform random graphs (integer weights 1..255), several
 sources each, sssp_collect_timings saves the solve
 state before every round and tries each of the 12 choices (push, all-edge,
 pull x four chunk widths): the round runs that choice and the rest of the
@@ -47,7 +47,8 @@ def collect(scales, sources_per_scale, edge_factor, seed0):
             f, t = g.collect_timings(int(s))
             X.append(f)
             T.append(t)
-            meta.append({"scale": scale, "seed": seed, "source": int(s), "rounds": len(f),
+            meta.append({"scale": scale, "edge_factor": edge_factor, "seed": seed,
+                         "source": int(s), "rounds": len(f),
                          "seconds": round(time.perf_counter() - t0, 2)})
             print(f"scale {scale} source {int(s)}: {len(f)} rounds", flush=True)
         g.close()
@@ -86,6 +87,21 @@ def labels(T):
     return ym, yc, t
 
 
+def stakes_weights(t, min_spread, max_copies):
+    """Rows whose choices all end within min_spread of each other carry no
+    usable label (timing noise decides) and are dropped (0 copies); the others
+    are repeated in proportion to what the choice is worth in milliseconds
+    (worst minus best remainder), so the loss weighs rounds by their stakes."""
+    flat = t.reshape(len(t), -1)
+    fin = np.where(np.isfinite(flat), flat, np.nan)
+    best, worst = np.nanmin(fin, axis=1), np.nanmax(fin, axis=1)
+    keep = worst > best * min_spread
+    stake = worst - best
+    med = np.median(stake[keep]) if keep.any() else 1.0
+    copies = np.clip(np.ceil(stake / max(med, 1e-9)), 1, max_copies).astype(np.int64)
+    return np.where(keep, copies, 0)
+
+
 def regret(t, pm, pc):
     """Total time of the chosen (mode, chunk) over the total of the best."""
     chosen = t[np.arange(len(t)), pm, pc]
@@ -97,6 +113,7 @@ def main():
     ap.add_argument("--scales", type=int, nargs="+", default=[19, 20, 21, 22, 23, 24])
     ap.add_argument("--sources", type=int, default=4)
     ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--ef32-scales", type=int, nargs="*", default=[19, 20, 21, 22])
     ap.add_argument("--er-logv", type=int, nargs="*", default=[20, 21, 22, 23])
     ap.add_argument("--er-degree", type=int, default=16)
     ap.add_argument("--seed", type=int, default=7)
@@ -104,30 +121,45 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=os.path.join(MODELS, "scheduler_mlp.f32"))
+    ap.add_argument("--data", help="train on the X/T arrays of an earlier run's _data.npz")
+    ap.add_argument("--min-spread", type=float, default=1.02)
+    ap.add_argument("--max-copies", type=int, default=8)
     args = ap.parse_args()
 
-    X, T, meta = collect(args.scales, args.sources, args.edge_factor, args.seed)
-    if args.er_logv:
-        Xe, Te, me = collect_er(args.er_logv, args.sources, args.er_degree, args.seed + 100)
-        X, T, meta = np.concatenate([X, Xe]), np.concatenate([T, Te]), meta + me
+    if args.data:
+        with np.load(args.data) as z:
+            X, T = z["X"], z["T"]
+        meta = [{"data": os.path.basename(args.data)}]
+    else:
+        X, T, meta = collect(args.scales, args.sources, args.edge_factor, args.seed)
+        if args.ef32_scales:
+            Xr, Tr, mr = collect(args.ef32_scales, args.sources, 32, args.seed + 50)
+            X, T, meta = np.concatenate([X, Xr]), np.concatenate([T, Tr]), meta + mr
+        if args.er_logv:
+            Xe, Te, me = collect_er(args.er_logv, args.sources, args.er_degree, args.seed + 100)
+            X, T, meta = np.concatenate([X, Xe]), np.concatenate([T, Te]), meta + me
     ym, yc, t = labels(T)
     n = len(X)
     rng = np.random.default_rng(args.seed)
     perm = rng.permutation(n)
     nval = max(1, n // 5)
     val, tr = perm[:nval], perm[nval:]
+    copies = stakes_weights(t[tr], args.min_spread, args.max_copies)
+    trw = np.repeat(tr, copies)
 
     mlp = S.MLP(0, seed=args.seed)
-    losses = mlp.train(X[tr], ym[tr], yc[tr], epochs=args.epochs, batch=min(args.batch, len(tr)),
-                       lr=args.lr, seed=args.seed)
+    losses = mlp.train(X[trw], ym[trw], yc[trw], epochs=args.epochs,
+                       batch=min(args.batch, len(trw)), lr=args.lr, seed=args.seed)
     out = mlp.forward(X)
     pm = out[:, :S.MLP_MODES].argmax(axis=1)
     pc = out[:, S.MLP_MODES:].argmax(axis=1)
     # the heuristic's choice on the same rows, for comparison (fixed chunks)
     report = {
         "rows": int(n), "train_rows": int(len(tr)), "val_rows": int(nval),
+        "train_rows_kept": int((copies > 0).sum()), "train_rows_weighted": int(len(trw)),
+        "min_spread": args.min_spread, "max_copies": args.max_copies,
         "scales": args.scales, "edge_factor": args.edge_factor, "sources_per_scale": args.sources,
-        "er_logv": args.er_logv, "er_degree": args.er_degree,
+        "ef32_scales": args.ef32_scales, "er_logv": args.er_logv, "er_degree": args.er_degree,
         "label_mode_counts": np.bincount(ym, minlength=S.MLP_MODES).tolist(),
         "final_train_loss": float(losses[-1]) if len(losses) else None,
         "val_mode_accuracy": float((pm[val] == ym[val]).mean()),

@@ -1,5 +1,7 @@
 """The workload configurations at full size, on the GPU.
 
+  C1   R-MAT scale 16, edge factor 16, one virtual generator thread (the
+       reference's own graph: seed 0); source 0
   C2   R-MAT scale 22, edge factor 32 (E = 134 M), base seed 1, 64 virtual
        generator threads; 16 sources from pick_sources(seed 2)
   C3   uniform random graph V = 2^24, average degree 16 (E = 268 M), srand(3),
@@ -13,9 +15,12 @@ At these sizes the oracle (Dijkstra, oracle/oracle.c) runs on a sample; every
 result is also checked through properties that hold at any size: the GPU
 certificate (no edge can still relax; a tight parent tree reaches every
 reached vertex), bit equality across schedules, and batched/packed solves
-equal to single solves. The timings printed are the solver's own device times.
+equal to single solves. The timings printed are the solver's own device times
+and, for C1-C3, the oracle's OpenMP restatement of cpu_bellman_ford on the host
+(all OpenMP threads, schedule(guided)), whose result is a second bit-exact check.
 """
 import os
+import time
 
 import numpy as np
 import pytest
@@ -37,6 +42,32 @@ def _with_network(g):
 
 def _gteps(E, ms):
     return E / (ms * 1e-3) / 1e9
+
+
+def _cpu_bf(oracle, src, dst, w, V, source):
+    t0 = time.perf_counter()
+    d, passes = oracle.bf(src, dst, w, V, source)
+    return d, passes, time.perf_counter() - t0
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(300)
+def test_c1_reference_graph_rmat16(gpu, oracle):
+    src, dst, w = S.rmat(16, 16, seed=1, threads=1)
+    V = 1 << 16
+    ref, passes, sec = _cpu_bf(oracle, src, dst, w, V, 0)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        m = _with_network(g)
+        t = {}
+        for sched in ("learned", "heuristic", "plain"):
+            g.solve(0, sched)
+            d, st = g.solve(0, sched)
+            assert np.array_equal(bits(d), bits(ref)), sched
+            t[sched] = round(_gteps(len(src), st["t_solve_ms"]), 3)
+        ok, counts, _ = g.certify(0, parents=False)
+        assert ok, counts
+        print("C1", t, "GTEPS; CPU", round(len(src) / sec / 1e9, 3), "GTEPS", passes, "passes")
+        del m
 
 
 @pytest.mark.gpu
@@ -63,7 +94,10 @@ def test_c2_rmat22_ef32_sixteen_sources(gpu, oracle):
         dj = oracle.dijkstra(src, dst, w, V, int(sources[0]))
         d, _ = g.solve(int(sources[0]), "learned")
         assert np.array_equal(bits(d), bits(dj))
-        print("C2", {k: round(_gteps(E * len(v), sum(v)), 2) for k, v in t.items()}, "GTEPS")
+        dc, passes, sec = _cpu_bf(oracle, src, dst, w, V, int(sources[0]))
+        assert np.array_equal(bits(dc), bits(d))
+        print("C2", {k: round(_gteps(E * len(v), sum(v)), 2) for k, v in t.items()}, "GTEPS; CPU",
+              round(E / sec / 1e9, 3), "GTEPS (1 source,", passes, "passes)")
         del m
 
 
@@ -86,7 +120,10 @@ def test_c3_er24_u32(gpu, oracle):
             assert np.array_equal(out[sched][0], out["learned"][0]), sched
         dj = oracle.dijkstra(src, dst, w, V, 0)
         assert np.array_equal(out["learned"][0], dj)
-        print("C3", {k: round(_gteps(len(src), v[1]), 2) for k, v in out.items()}, "GTEPS")
+        dc, passes, sec = _cpu_bf(oracle, src, dst, w, V, 0)
+        assert np.array_equal(dc, dj)
+        print("C3", {k: round(_gteps(len(src), v[1]), 2) for k, v in out.items()}, "GTEPS; CPU",
+              round(len(src) / sec / 1e9, 3), "GTEPS", passes, "passes")
         del m
 
 
