@@ -113,6 +113,20 @@ int sssp_graph_create_rmat(int32_t scale, int32_t edge_factor, int32_t a, int32_
                            int32_t d, uint32_t base_seed, int32_t virtual_threads, int32_t device,
                            uint32_t flags, int32_t *src_out, int32_t *dst_out, float *w_out,
                            sssp_graph **out);
+/* Device ingest of a binary edge-list file (the reference's format: 1-byte
+ * directed flag, int32 vertex count, int64 edge count, then packed records of
+ * int32 src, int32 dst and a 4-byte weight). The records are read in chunks
+ * into two pinned host buffers while the previous chunk is copied and split
+ * into the edge arrays on the device; the layouts are then built there. No
+ * host-side conversion or full host copy of the graph is made.
+ * `directed` must match the file's flag (as the reference throws on a
+ * mismatch); weight_type is SSSP_W_F32 or SSSP_W_U32 (files with 8-byte
+ * weights are rejected: device graphs hold 4-byte weights).
+ * Replaces GraphStorageAPI::load_from_edges_list_bin_file
+ * (include/storage/graph_storage_API.hpp:113-157) followed by the upload of
+ * gpu_bellman_ford (algorithm/sssp/sssp.hpp:245-253). */
+int sssp_graph_create_from_bin(const char *path, int32_t directed, int32_t weight_type,
+                               int32_t device, uint32_t flags, sssp_graph **out);
 int sssp_graph_destroy(sssp_graph *g);
 int sssp_graph_info(const sssp_graph *g, int32_t *vertices, int64_t *edges,
                     int32_t *weight_type, int64_t *device_bytes);

@@ -219,6 +219,91 @@ int sssp_graph_create(int32_t weight_type, const int32_t *src, const int32_t *ds
     return rc;
 }
 
+int sssp_graph_create_from_bin(const char *path, int32_t directed, int32_t weight_type,
+                               int32_t device, uint32_t flags, sssp_graph **out)
+{
+    sssp_graph *g = nullptr;
+    FILE *f = nullptr;
+    struct Host {
+        void *p[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Host()
+        {
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+            for (void *x : p)
+                if (x) (void)hipHostFree(x);
+        }
+    };
+    int rc = guard([&] {
+        SSSP_REQUIRE(out && path, "null argument");
+        *out = nullptr;
+        f = std::fopen(path, "rb");
+        if (!f) throw Error{SSSP_ERR_IO, std::string("cannot open ") + path};
+        uint8_t dir = 0;
+        int32_t V = 0;
+        int64_t E = 0;
+        if (std::fread(&dir, 1, 1, f) != 1 || std::fread(&V, 4, 1, f) != 1 ||
+            std::fread(&E, 8, 1, f) != 1)
+            throw Error{SSSP_ERR_IO, std::string("truncated header in ") + path};
+        if (V < 1 || E < 0) throw Error{SSSP_ERR_IO, std::string("corrupt header in ") + path};
+        if ((dir != 0) != (directed != 0))
+            throw Error{SSSP_ERR_ARG, "loading a directed graph as undirected (or undirected as directed)"};
+        if (std::fseek(f, 0, SEEK_END) != 0)
+            throw Error{SSSP_ERR_IO, std::string("cannot seek in ") + path};
+        const long size = std::ftell(f);
+        const int64_t body = (int64_t)size - 13;
+        if (E > 0 && body == 16 * E)
+            throw Error{SSSP_ERR_ARG, "file has 8-byte weights; device graphs hold 4-byte weights"};
+        if (body != 12 * E)
+            throw Error{SSSP_ERR_IO, std::string("edge records do not match the edge count in ") + path};
+        if (std::fseek(f, 13, SEEK_SET) != 0)
+            throw Error{SSSP_ERR_IO, std::string("cannot seek in ") + path};
+        g = graph_begin(weight_type, V, E, device);
+        hipStream_t s = g->stream;
+        DevBuf arr, stage;
+        int32_t *d_src = dmalloc<int32_t>((size_t)(E > 0 ? E : 1));
+        arr.p[0] = d_src;
+        int32_t *d_dst = dmalloc<int32_t>((size_t)(E > 0 ? E : 1));
+        arr.p[1] = d_dst;
+        uint32_t *d_w = dmalloc<uint32_t>((size_t)(E > 0 ? E : 1));
+        arr.p[2] = d_w;
+        const int64_t chunk = E < ((int64_t)1 << 22) ? (E > 0 ? E : 1) : ((int64_t)1 << 22);
+        Host h;
+        for (int b = 0; b < 2; b++) {
+            stage.p[b] = dmalloc<uint32_t>((size_t)(3 * chunk));
+            SSSP_HIP(hipHostMalloc(&h.p[b], (size_t)(12 * chunk), hipHostMallocDefault));
+            SSSP_HIP(hipEventCreate(&h.ev[b]));
+        }
+        // host reads of chunk k+1 overlap the copy and split of chunk k
+        try {
+            for (int64_t e0 = 0, k = 0; e0 < E; e0 += chunk, k++) {
+                const int b = (int)(k & 1);
+                const int64_t n = E - e0 < chunk ? E - e0 : chunk;
+                if (k >= 2) SSSP_HIP(hipEventSynchronize(h.ev[b]));  // buffer b free again
+                if (std::fread(h.p[b], 12, (size_t)n, f) != (size_t)n)
+                    throw Error{SSSP_ERR_IO, std::string("truncated edge records in ") + path};
+                uint32_t *dst_stage = static_cast<uint32_t *>(stage.p[b]);
+                SSSP_HIP(hipMemcpyAsync(dst_stage, h.p[b], (size_t)(12 * n), hipMemcpyHostToDevice,
+                                        s));
+                split_records(dst_stage, n, d_src + e0, d_dst + e0, d_w + e0, s);
+                SSSP_HIP(hipEventRecord(h.ev[b], s));
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(s);  // no copy may still read the pinned buffers
+            throw;
+        }
+        std::fclose(f);
+        f = nullptr;
+        SSSP_HIP(hipStreamSynchronize(s));
+        graph_finish(g, d_src, d_dst, d_w, flags & ~SSSP_INPUT_DEVICE);
+        *out = g;
+    });
+    if (f) std::fclose(f);
+    if (rc != SSSP_OK && g) graph_free(g);
+    return rc;
+}
+
 int sssp_graph_create_rmat(int32_t scale, int32_t edge_factor, int32_t a, int32_t b, int32_t c,
                            int32_t d, uint32_t base_seed, int32_t virtual_threads, int32_t device,
                            uint32_t flags, int32_t *src_out, int32_t *dst_out, float *w_out,

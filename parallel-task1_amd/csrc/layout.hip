@@ -401,6 +401,27 @@ void build_tile_table(const int64_t *scan, int32_t n, int64_t total, int32_t til
     SSSP_LAUNCH_CHECK("k_tile_table");
 }
 
+// Packed 12-byte edge records (src, dst, 4-byte weight) of the binary
+// edge-list file -> the three edge arrays.
+__global__ __launch_bounds__(kBlock) void k_split_records(const uint32_t *__restrict__ rec, int64_t n,
+                                                         int32_t *src, int32_t *dst, uint32_t *w)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        src[i] = (int32_t)rec[3 * i];
+        dst[i] = (int32_t)rec[3 * i + 1];
+        w[i] = rec[3 * i + 2];
+    }
+}
+
+void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, uint32_t *w,
+                   hipStream_t s)
+{
+    if (n <= 0) return;
+    k_split_records<<<grid_for(n), kBlock, 0, s>>>(rec, n, src, dst, w);
+    SSSP_LAUNCH_CHECK("k_split_records");
+}
+
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
                    bool build_csc)
 {

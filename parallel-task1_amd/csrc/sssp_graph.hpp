@@ -207,6 +207,9 @@ void rmat_device(int scale, int ef, int a, int b, int c, int d, uint32_t base_se
 
 // layout.hip
 void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s);
+// packed 12-byte binary edge-list records (device) -> src, dst, w (device)
+void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, uint32_t *w,
+                   hipStream_t s);
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
                    bool build_csc);
 void device_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *blk_a,

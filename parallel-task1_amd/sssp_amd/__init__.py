@@ -66,6 +66,7 @@ _SIGS = {
     "sssp_device_count": (I32, [PI32]),
     "sssp_device_info": (I32, [I32, C.c_char_p, I32, PI32, PI64, PI32]),
     "sssp_graph_create": (I32, [I32, P, P, P, I32, I64, I32, U32, C.POINTER(P)]),
+    "sssp_graph_create_from_bin": (I32, [C.c_char_p, I32, I32, I32, U32, C.POINTER(P)]),
     "sssp_graph_create_rmat": (I32, [I32, I32, I32, I32, I32, I32, U32, I32, I32, U32, P, P, P,
                                      C.POINTER(P)]),
     "sssp_graph_destroy": (I32, [P]),
@@ -248,6 +249,29 @@ class DeviceGraph:
         self.wtype, self.wdtype = W_F32, np.float32
         self.V, self.E = 1 << scale, E
         return (self, arrs) if download_edges else self
+
+    @classmethod
+    def from_bin(cls, path, directed=True, weight_dtype=np.float32, device=0, csc=True):
+        """Device ingest of a binary edge-list file (the reference's format):
+        chunks stream through pinned buffers onto the device, where the
+        layouts are built; no host copy of the graph is made."""
+        self = cls.__new__(cls)
+        if np.dtype(weight_dtype) == np.float32:
+            self.wtype, self.wdtype = W_F32, np.float32
+        elif np.dtype(weight_dtype) == np.uint32:
+            self.wtype, self.wdtype = W_U32, np.uint32
+        else:
+            raise SSSPError(ERR_ARG, f"unsupported weight dtype {weight_dtype}")
+        h = C.c_void_p()
+        _check(lib().sssp_graph_create_from_bin(str(path).encode(), 1 if directed else 0,
+                                                self.wtype, device, 0 if csc else NO_CSC,
+                                                C.byref(h)))
+        self._h = h
+        self.device = device
+        V, E = C.c_int32(), C.c_int64()
+        _check(lib().sssp_graph_info(h, C.byref(V), C.byref(E), None, None))
+        self.V, self.E = V.value, E.value
+        return self
 
     def __init__(self, src, dst, w, vertices: int, device: int = 0, csc: bool = True):
         w = np.ascontiguousarray(w)

@@ -44,6 +44,41 @@ def test_small_graphs_every_schedule(gpu, oracle, name, V, src, dst, w):
 
 
 @pytest.mark.gpu
+def test_device_ingest_from_bin(gpu, oracle, tmp_path):
+    """A graph streamed from the reference's binary file format equals the one
+    built from the same arrays: layouts, and distances under every schedule."""
+    src, dst, w = S.rmat(14, 16, seed=12, threads=8)
+    V = 1 << 14
+    p = tmp_path / "g.bin"
+    S.save_bin(p, src, dst, w, V)
+    with S.DeviceGraph.from_bin(p) as g1, S.DeviceGraph(src, dst, w, V) as g2:
+        assert (g1.V, g1.E) == (V, len(src))
+        L1, L2 = g1.layout(), g2.layout()
+        for k in L1:
+            assert np.array_equal(L1[k].view(np.uint8), L2[k].view(np.uint8)), k
+        ref = oracle.dijkstra(src, dst, w, V, 3)
+        for sched in SCHEDS:
+            d, _ = g1.solve(3, sched)
+            assert np.array_equal(bits(d), bits(ref)), sched
+    su, tu, wu = S.er(1 << 12, 8, seed=5, weight_mode=1)
+    pu = tmp_path / "u.bin"
+    S.save_bin(pu, su, tu, wu, 1 << 12)
+    with S.DeviceGraph.from_bin(pu, weight_dtype=np.uint32) as g:
+        d, _ = g.solve(0, "heuristic")
+        assert np.array_equal(d, oracle.dijkstra(su, tu, wu, 1 << 12, 0))
+    # several chunks (4 Mi records each) through the two pinned buffers
+    s2, t2, w2 = S.rmat(20, 16, seed=13, threads=64)
+    p2 = tmp_path / "big.bin"
+    S.save_bin(p2, s2, t2, w2, 1 << 20)
+    with S.DeviceGraph.from_bin(p2, csc=False) as g1, S.DeviceGraph(s2, t2, w2, 1 << 20, csc=False) as g2:
+        a, _ = g1.solve(5, "plain")
+        b, _ = g2.solve(5, "plain")
+        assert np.array_equal(bits(a), bits(b))
+        L1, L2 = g1.layout(csc=False), g2.layout(csc=False)
+        assert np.array_equal(L1["el_dst"], L2["el_dst"])
+
+
+@pytest.mark.gpu
 def test_layout_is_stable_sort(gpu):
     src, dst, w = S.rmat(11, 16, seed=3, threads=4)
     V = 1 << 11

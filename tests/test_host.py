@@ -100,6 +100,40 @@ def test_bin_roundtrip_and_errors(tmp_path):
     assert e.value.code == S.ERR_IO
 
 
+def test_device_ingest_rejects_bad_files_before_any_device_call(tmp_path):
+    """sssp_graph_create_from_bin checks the file (open, header, directed flag,
+    record size) before it touches the GPU, so these errors need none."""
+    s, t, w = S.rmat(8, 4, seed=3, threads=2)
+    p = tmp_path / "r.bin"
+    S.save_bin(p, s, t, w, 256)
+    raw = p.read_bytes()
+    assert len(raw) == 13 + 12 * len(s)
+
+    def code(path, **kw):
+        with pytest.raises(S.SSSPError) as e:
+            S.DeviceGraph.from_bin(path, **kw)
+        return e.value.code
+
+    assert code(tmp_path / "missing.bin") == S.ERR_IO
+    (tmp_path / "trunc.bin").write_bytes(raw[:-5])
+    assert code(tmp_path / "trunc.bin") == S.ERR_IO
+    (tmp_path / "tail.bin").write_bytes(raw + b"abc")
+    assert code(tmp_path / "tail.bin") == S.ERR_IO
+    (tmp_path / "hdr.bin").write_bytes(raw[:7])
+    assert code(tmp_path / "hdr.bin") == S.ERR_IO
+    (tmp_path / "v0.bin").write_bytes(raw[:1] + np.int32(0).tobytes() + raw[5:])
+    assert code(tmp_path / "v0.bin") == S.ERR_IO
+    # the reference throws when the directed flag differs from the container's
+    assert code(p, directed=False) == S.ERR_ARG
+    S.save_bin(tmp_path / "und.bin", s, t, w, 256, directed=False)
+    assert code(tmp_path / "und.bin") == S.ERR_ARG
+    # 8-byte weights: the device graph holds 4-byte weights
+    S.save_bin(tmp_path / "f64.bin", s, t, w.astype(np.float64), 256)
+    assert code(tmp_path / "f64.bin") == S.ERR_ARG
+    with pytest.raises(S.SSSPError):
+        S.DeviceGraph.from_bin(p, weight_dtype=np.float64)
+
+
 def test_storage_api_mirror(tmp_path):
     g = S.GraphGenerationAPI.R_MAT_parallel(1 << 8, 4, 57, 19, 19, 5, 1, True)
     p = tmp_path / "m.bin"
