@@ -167,6 +167,26 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
 int sssp_set_param(sssp_graph *g, const char *name, double value);
 int sssp_get_param(const sssp_graph *g, const char *name, double *value);
 
+/* ---- BFS on the same engine ----------------------------------------------- */
+/* Breadth-first search from `source` over the graph's edges (weights
+ * ignored), run by the same round engine as the SSSP with every edge counting
+ * one hop: the fixpoint of lev[v] <= lev[u] + 1 is the hop distance. Any
+ * schedule may be used (push rounds are top-down, pull rounds bottom-up, the
+ * rule or the network picks per round). visited_out (host, V bytes, may be
+ * NULL) receives the reference's result, 1 for every vertex reachable from
+ * source, else 0; levels_out (host, V, may be NULL) the hop levels
+ * (0xFFFFFFFF unreached). sssp_certify then checks the levels (every edge
+ * satisfies lev[v] <= lev[u] + 1, a tight parent tree reaches every visited
+ * vertex). Replaces BFS::gpu_bfs (algorithm/bfs/bfs.hpp:213-253). */
+int sssp_bfs(sssp_graph *g, int32_t source, int32_t schedule, uint8_t *visited_out,
+             uint32_t *levels_out, sssp_stats *stats);
+/* bfs_wrapper (algorithm/bfs/gpu/bfs.cu:44-85) over caller-owned device
+ * arrays: d_visited (device, V bytes) = 1 for every vertex reachable from
+ * source over the edges (d_src[i] -> d_dst[i]), after all-edge passes until
+ * one marks nothing; *passes (may be NULL) = passes run. */
+int sssp_bfs_edges(const int32_t *d_src, const int32_t *d_dst, int32_t vertices, int64_t edges,
+                   uint8_t *d_visited, int32_t source, int32_t *passes);
+
 /* ---- batched solves (one persistent launch, replayed through a hipGraph) -- */
 /* Many sources on one graph. dist_out: n*V elements (host) or NULL.
  * policy 0 = autotuned, 1 = one workgroup per source, 2 = merged frontier. */

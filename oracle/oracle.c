@@ -17,6 +17,9 @@
  *   orc_certify_f32/_u32      -- shortest-path certificate: every edge satisfied,
  *                                d[s] == 0, and a tight parent tree rooted at s
  *                                for every reachable vertex
+ *   orc_bfs                   -- BFS::cpu_sequential_bfs, algorithm/bfs/bfs.hpp:3-45
+ *                                (queue over the CSR, reachability from the
+ *                                source), plus the hop level of every vertex
  *
  * Pins (tests/test_oracle.py): the reference's own cpu_bellman_ford compiled
  * from /root/reference into oracle/_ref (golden vectors in tests/golden/),
@@ -416,6 +419,41 @@ int orc_certify_u32(const int32_t *src, const int32_t *dst, const uint32_t *w,
                     const int32_t *parent, int64_t *counts)
 {
     return certify_generic(src, dst, w, V, E, source, dist, parent, 1, counts);
+}
+
+/* ---------------------------------------------------------------------------
+ * BFS (BFS::cpu_sequential_bfs, algorithm/bfs/bfs.hpp:3-45): FIFO queue over
+ * the out-edges in CSR order, a vertex is marked when first seen. visited[v]
+ * is the reference's bool result (1/0); level[v] the hop count of the first
+ * discovery (0xFFFFFFFF when unreached), which a FIFO BFS makes the shortest
+ * hop count. Returns the number of reached vertices, -1 on bad input.
+ * ------------------------------------------------------------------------- */
+int64_t orc_bfs(const int32_t *src, const int32_t *dst, int32_t V, int64_t E, int32_t source,
+                uint8_t *visited, uint32_t *level)
+{
+    if (V < 1 || source < 0 || source >= V || E < 0) return -1;
+    int64_t *perm = NULL;
+    int64_t *rp = build_row_ptr(src, V, E, &perm);
+    int32_t *q = (int32_t *)malloc((size_t)V * sizeof(int32_t));
+    if (!rp || !q) { free(rp); free(perm); free(q); return -1; }
+    for (int32_t v = 0; v < V; v++) { visited[v] = 0; if (level) level[v] = 0xFFFFFFFFu; }
+    int64_t head = 0, tail = 0;
+    q[tail++] = source;
+    visited[source] = 1;
+    if (level) level[source] = 0;
+    while (head < tail) {
+        const int32_t u = q[head++];
+        for (int64_t i = rp[u]; i < rp[u + 1]; i++) {
+            const int32_t t = dst[perm[i]];
+            if (!visited[t]) {
+                visited[t] = 1;
+                if (level) level[t] = level[u] + 1;
+                q[tail++] = t;
+            }
+        }
+    }
+    free(rp); free(perm); free(q);
+    return tail;
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }

@@ -14,10 +14,12 @@
 #include "include/test/generation/graph_generation_API.h"
 #include "include/storage/graph_storage_API.h"
 #include "algorithm/sssp/sssp.h"
+#include "algorithm/bfs/bfs.h"
 
 #include <cstdint>
 #include <cstring>
 #include <cstdlib>
+#include <vector>
 
 template <typename W>
 static void copy_out(Graph<int, W> &g, int32_t *src, int32_t *dst, W *w)
@@ -119,6 +121,25 @@ int ref_load_bin_f32(const char *path, int32_t *V_out, int64_t *E_out,
         *E_out = g.get_edges_count();
         if (src) copy_out(g, src, dst, w);
     } catch (const char *) { return 2; }
+    return 0;
+}
+
+// BFS::cpu_sequential_bfs (algorithm/bfs/bfs.hpp:3-45): the reference's bool
+// reachability result, returned as 0/1 bytes. The graph is filled as
+// load_from_edges_list_bin_file does (weights unused by BFS: 1).
+int ref_cpu_sequential_bfs(const int32_t *src, const int32_t *dst, int32_t V, int64_t E,
+                           int32_t source, uint8_t *visited)
+{
+    try {
+        Graph<int, float> g(0, 0, EDGES_LIST, true);
+        std::vector<float> w((size_t)(E > 0 ? E : 1), 1.0f);
+        fill_graph(g, src, dst, w.data(), V, E);
+        bool *res = new bool[(size_t)V];
+        BFS<int, float> op(1);
+        op.cpu_sequential_bfs(g, res, source);
+        for (int32_t i = 0; i < V; i++) visited[i] = res[i] ? 1 : 0;
+        delete[] res;
+    } catch (const char *) { return 1; }
     return 0;
 }
 

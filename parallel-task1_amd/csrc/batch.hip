@@ -450,6 +450,10 @@ int sssp_solve_batch(sssp_graph *g, const int32_t *sources, int32_t n, int32_t p
         for (int32_t i = 0; i < n; i++)
             SSSP_REQUIRE(sources[i] >= 0 && sources[i] < g->V, "source vertex out of range");
         SSSP_HIP(hipSetDevice(g->device));
+        // the merged/sequential policy solves through g->dist: no single-solve
+        // result (distances or BFS levels) survives a batch
+        g->have_result = false;
+        g->bfs_result = false;
         Config c = default_config(g, n);
         if (policy != 0 && policy != c.policy) c = (policy == 1) ? Config{1, 1024, 1, 0.05, 1}
                                                                  : Config{2, 256, 2, g->push_edge_frac, g->push_chunk};

@@ -390,6 +390,7 @@ int sssp_solve(sssp_graph *g, int32_t source, int32_t schedule, void *dist_out, 
         sssp_stats st;
         std::memset(&st, 0, sizeof(st));
         g->have_result = false;
+        g->bfs_result = false;
         solve(g, &source, 1, schedule, &st);
         g->last_source = source;
         g->last_sched = schedule;
@@ -397,6 +398,47 @@ int sssp_solve(sssp_graph *g, int32_t source, int32_t schedule, void *dist_out, 
         if (dist_out)
             SSSP_HIP(hipMemcpy(dist_out, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
         if (stats) *stats = st;
+    });
+}
+
+int sssp_bfs(sssp_graph *g, int32_t source, int32_t schedule, uint8_t *visited_out,
+             uint32_t *levels_out, sssp_stats *stats)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g, "null graph");
+        SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
+        SSSP_REQUIRE(schedule >= SSSP_SCHED_PLAIN && schedule <= SSSP_SCHED_PULL, "unknown schedule");
+        set_device(g->device);
+        sssp_stats st;
+        std::memset(&st, 0, sizeof(st));
+        g->have_result = false;
+        g->bfs_result = false;
+        bfs(g, source, schedule, &st);
+        g->last_source = source;
+        g->last_sched = schedule;
+        g->bfs_result = true;
+        if (levels_out)
+            SSSP_HIP(hipMemcpy(levels_out, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+        if (visited_out) {
+            uint8_t *dv = dmalloc<uint8_t>((size_t)g->V);
+            DevBuf own;
+            own.p[0] = dv;
+            levels_to_visited(g->dist, g->V, dv, g->stream);
+            SSSP_HIP(hipStreamSynchronize(g->stream));
+            SSSP_HIP(hipMemcpy(visited_out, dv, (size_t)g->V, hipMemcpyDeviceToHost));
+        }
+        if (stats) *stats = st;
+    });
+}
+
+int sssp_bfs_edges(const int32_t *d_src, const int32_t *d_dst, int32_t vertices, int64_t edges,
+                   uint8_t *d_visited, int32_t source, int32_t *passes)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_visited && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst)),
+                     "bad arguments");
+        SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        ref_bfs(d_src, d_dst, vertices, edges, d_visited, source, passes);
     });
 }
 
@@ -432,10 +474,11 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
 {
     return guard([&] {
         SSSP_REQUIRE(g && counts, "null argument");
-        SSSP_REQUIRE(g->have_result, "no solve result on this graph");
+        SSSP_REQUIRE(g->have_result || g->bfs_result, "no solve result on this graph");
         SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
         set_device(g->device);
-        certify(g, &source, 1, counts);
+        if (g->bfs_result) certify_levels(g, source, counts);  // hop levels: every edge weighs 1
+        else certify(g, &source, 1, counts);
         if (parent_out)
             SSSP_HIP(hipMemcpy(parent_out, g->parent, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
         if (valid) *valid = (counts[0] == 0 && counts[1] == 0 && counts[2] == 0) ? 1 : 0;
@@ -569,6 +612,7 @@ int sssp_collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feat
         SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
         set_device(g->device);
         g->have_result = false;
+        g->bfs_result = false;
         collect_timings(g, source, cap, features, times_ms, rows);
         g->last_source = source;
         g->have_result = true;

@@ -1653,6 +1653,16 @@ void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp
     else solve_t<WU32>(g, sources, n_src, sched, st);
 }
 
+void bfs(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
+{
+    solve_t<WUnit>(g, &source, 1, sched, st);
+}
+
+void certify_levels(sssp_graph *g, int32_t source, int64_t *counts)
+{
+    certify_t<WUnit>(g, &source, 1, counts);
+}
+
 void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
                      int32_t *rows)
 {
@@ -1723,6 +1733,39 @@ void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int6
             throw Error{SSSP_ERR_NOT_CONVERGED, "Bellman-Ford did not converge within V passes"};
     }
     if (passes_out) *passes_out = done_at + 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_levels_to_visited(const uint32_t *lev, int64_t V,
+                                                              uint8_t *vis)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * kBlock)
+        vis[v] = lev[v] != WUnit::INF ? 1 : 0;
+}
+
+void levels_to_visited(const uint32_t *lev, int32_t V, uint8_t *vis, hipStream_t s)
+{
+    k_levels_to_visited<<<grid_for(V), kBlock, 0, s>>>(lev, V, vis);
+    SSSP_LAUNCH_CHECK("k_levels_to_visited");
+}
+
+// The reference's bfs_wrapper (bfs.cu:44-85) over caller-owned device edge
+// arrays: all-edge passes until one marks nothing. The passes run on levels
+// (unit-weight Bellman-Ford, the dst array stands in for the unused weights)
+// and end in the visited flags, 1 for every vertex reachable from source.
+void ref_bfs(const int32_t *src, const int32_t *dst, int32_t V, int64_t E, uint8_t *visited,
+             int32_t source, int32_t *passes_out)
+{
+    uint32_t *lev = dmalloc<uint32_t>((size_t)V);
+    struct Free {
+        void *p;
+        ~Free() { dfree(p); }
+    } fr{lev};
+    ref_init<WUnit, uint32_t>(lev, V, source);
+    ref_bf<WUnit, uint32_t>(src, dst, reinterpret_cast<const uint32_t *>(dst), V, E, lev,
+                            passes_out);
+    levels_to_visited(lev, V, visited, nullptr);
+    SSSP_HIP(hipDeviceSynchronize());
 }
 
 template void ref_init<WF32, uint32_t>(uint32_t *, int32_t, int32_t);

@@ -147,7 +147,8 @@ struct sssp_graph {
     // last solve
     int32_t last_source = -1;
     int last_sched = -1;
-    bool have_result = false;
+    bool have_result = false;     // g->dist holds SSSP distances of last_source
+    bool bfs_result = false;      // g->dist holds BFS levels of last_source
     std::vector<sssp::TraceRow> trace;
 
     sssp_mlp *mlp = nullptr;
@@ -224,6 +225,12 @@ void ref_init(DT *d, int32_t V, int32_t source);
 template <class WT, class DT>
 void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int64_t E, DT *d,
             int32_t *passes_out);
+// BFS on the graph's edges (weights ignored): hop levels in g->dist, any schedule
+void bfs(sssp_graph *g, int32_t source, int sched, sssp_stats *st);
+void certify_levels(sssp_graph *g, int32_t source, int64_t *counts);
+void levels_to_visited(const uint32_t *lev, int32_t V, uint8_t *vis, hipStream_t s);
+void ref_bfs(const int32_t *src, const int32_t *dst, int32_t V, int64_t E, uint8_t *visited,
+             int32_t source, int32_t *passes_out);
 void reset_stamps(sssp_graph *g);  // stamps: 0 (out-edges) / kSinkStamp; pile states: 0
 void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp_stats *st);
 void certify(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *counts);

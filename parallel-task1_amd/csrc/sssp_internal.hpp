@@ -117,6 +117,14 @@ struct WU32 {
     }
 };
 
+// BFS levels: every edge is one hop and its weight is ignored; the fixpoint
+// of d[v] <= d[u] + 1 is the hop distance, and d != INF is the reference's
+// visited flag (bfs.cu:25-39). Callers never add to INF; levels stay < V.
+struct WUnit {
+    static constexpr uint32_t INF = 0xFFFFFFFFu;
+    __device__ __forceinline__ static uint32_t add(uint32_t d, uint32_t) { return d + 1u; }
+};
+
 // 64-bit distances (f64 reference-shaped wrapper only).
 struct WF64 {
     static constexpr unsigned long long INF = 0x7FEFFFFFFFFFFFFFull;  // DBL_MAX

@@ -11,6 +11,8 @@
 //   SingleSourceShortestPaths<V,W>::gpu_bellman_ford  (algorithm/sssp/sssp.h:23-41)
 //   init_distances_wrapper<W> / bellman_ford_wrapper<W>
 //                           (algorithm/sssp/gpu/bellman_ford.cuh:13-18)
+//   BFS<V,W>::gpu_bfs and bfs_wrapper (algorithm/bfs/bfs.h:14-45,
+//                           algorithm/bfs/gpu/bfs.cuh:12)
 // Everything device-side goes through the C ABI of libsssp_mi355x.so
 // (include/sssp_mi355x.h). Failures throw `const char*`, as the reference's
 // SAFE_CALL does (include/common/cuda_error_hadling.h:4-24).
@@ -306,6 +308,54 @@ inline void bellman_ford_wrapper<unsigned int>(int *s, int *t, unsigned int *w, 
     sssp_host::check(sssp_bellman_ford_u32(s, t, w, V, E, src, d, NULL));
 }
 
+namespace sssp_host {
+// The device graph of the last Graph passed in, rebuilt only when its data
+// pointers or sizes change (the reference uploaded on every call,
+// sssp.hpp:245-253 and bfs.hpp:230-238).
+template <typename _TVertexValue, typename _TEdgeWeight>
+class DeviceGraphCache {
+  public:
+    DeviceGraphCache() {}
+    ~DeviceGraphCache()
+    {
+        if (g_) sssp_graph_destroy(g_);
+    }
+    DeviceGraphCache(const DeviceGraphCache &) = delete;
+    DeviceGraphCache &operator=(const DeviceGraphCache &) = delete;
+
+    sssp_graph *get(Graph<_TVertexValue, _TEdgeWeight> &graph, int device, sssp_mlp *mlp)
+    {
+        graph.convert_to_edges_list();
+        GraphContainerData<_TVertexValue, _TEdgeWeight> d = graph.get_graph_data();
+        const int V = graph.get_vertices_count();
+        const long long E = graph.get_edges_count();
+        if (!g_ || key_src_ != d.edges_src_ids || key_dst_ != d.edges_dst_ids ||
+            key_w_ != (const void *)d.edges_weights || key_v_ != V || key_e_ != E) {
+            if (g_) sssp_graph_destroy(g_);
+            g_ = NULL;
+            check(sssp_graph_create(WeightType<_TEdgeWeight>::code, d.edges_src_ids,
+                                    d.edges_dst_ids, d.edges_weights, V, E, device, 0, &g_));
+            key_src_ = d.edges_src_ids;
+            key_dst_ = d.edges_dst_ids;
+            key_w_ = d.edges_weights;
+            key_v_ = V;
+            key_e_ = E;
+        }
+        if (mlp) check(sssp_graph_set_mlp(g_, mlp));
+        return g_;
+    }
+    sssp_graph *current() const { return g_; }
+
+  private:
+    sssp_graph *g_ = NULL;
+    const int *key_src_ = NULL;
+    const int *key_dst_ = NULL;
+    const void *key_w_ = NULL;
+    int key_v_ = 0;
+    long long key_e_ = 0;
+};
+}  // namespace sssp_host
+
 // SingleSourceShortestPaths<V,W>::gpu_bellman_ford (sssp.hpp:219-278): the
 // graph is uploaded once and cached (keyed on its data pointers and sizes),
 // the solve runs on the device, the distances are copied to `distances`, and
@@ -319,7 +369,6 @@ class SingleSourceShortestPaths {
         : omp_threads_(omp_threads), device_(device), schedule_(schedule) {}
     ~SingleSourceShortestPaths()
     {
-        if (g_) sssp_graph_destroy(g_);
         if (mlp_) sssp_mlp_destroy(mlp_);
     }
     SingleSourceShortestPaths(const SingleSourceShortestPaths &) = delete;
@@ -333,44 +382,61 @@ class SingleSourceShortestPaths {
     {
         if (!mlp_) sssp_host::check(sssp_mlp_create(device_, 0, &mlp_));
         sssp_host::check(sssp_mlp_set_params(mlp_, params));
-        if (g_) sssp_host::check(sssp_graph_set_mlp(g_, mlp_));
+        if (cache_.current()) sssp_host::check(sssp_graph_set_mlp(cache_.current(), mlp_));
     }
 
     double gpu_bellman_ford(Graph<_TVertexValue, _TEdgeWeight> &graph, _TEdgeWeight *distances,
                             int source_vertex)
     {
-        graph.convert_to_edges_list();
-        GraphContainerData<_TVertexValue, _TEdgeWeight> d = graph.get_graph_data();
-        const int V = graph.get_vertices_count();
-        const long long E = graph.get_edges_count();
-        if (!g_ || key_src_ != d.edges_src_ids || key_dst_ != d.edges_dst_ids ||
-            key_w_ != (const void *)d.edges_weights || key_v_ != V || key_e_ != E) {
-            if (g_) sssp_graph_destroy(g_);
-            g_ = NULL;
-            sssp_host::check(sssp_graph_create(sssp_host::WeightType<_TEdgeWeight>::code,
-                                               d.edges_src_ids, d.edges_dst_ids, d.edges_weights, V,
-                                               E, device_, 0, &g_));
-            if (mlp_) sssp_host::check(sssp_graph_set_mlp(g_, mlp_));
-            key_src_ = d.edges_src_ids;
-            key_dst_ = d.edges_dst_ids;
-            key_w_ = d.edges_weights;
-            key_v_ = V;
-            key_e_ = E;
-        }
+        sssp_graph *g = cache_.get(graph, device_, mlp_);
         sssp_stats st;
-        sssp_host::check(sssp_solve(g_, source_vertex, schedule_, distances, &st));
-        return E / (st.t_solve_ms * 1e-3 * 1e6);
+        sssp_host::check(sssp_solve(g, source_vertex, schedule_, distances, &st));
+        return graph.get_edges_count() / (st.t_solve_ms * 1e-3 * 1e6);
     }
 
   private:
     int omp_threads_;
     int device_;
     int schedule_;
-    sssp_graph *g_ = NULL;
     sssp_mlp *mlp_ = NULL;
-    const int *key_src_ = NULL;
-    const int *key_dst_ = NULL;
-    const void *key_w_ = NULL;
-    int key_v_ = 0;
-    long long key_e_ = 0;
+    sssp_host::DeviceGraphCache<_TVertexValue, _TEdgeWeight> cache_;
+};
+
+// bfs_wrapper (algorithm/bfs/gpu/bfs.cuh:12, bfs.cu:44-85) on caller-owned
+// device arrays: visited (device, bool per vertex) = reachable from source.
+inline void bfs_wrapper(int *src_ids, int *dst_ids, int vertices_count, long long edges_count,
+                        bool *visited, int source_vertex)
+{
+    static_assert(sizeof(bool) == 1, "bool must be one byte, as the visited array");
+    sssp_host::check(sssp_bfs_edges(src_ids, dst_ids, vertices_count, edges_count,
+                                    reinterpret_cast<uint8_t *>(visited), source_vertex, NULL));
+}
+
+// BFS<V,W>::gpu_bfs (algorithm/bfs/bfs.hpp:213-253): reachability from the
+// source into `result`, run by the same device engine as the SSSP on the
+// cached graph; prints and returns MTEPS over the solve window as the
+// reference's "no memcpy performance" line (bfs.hpp:244) measures it.
+template <typename _TVertexValue, typename _TEdgeWeight>
+class BFS {
+  public:
+    explicit BFS(int omp_threads = 1, int device = 0, int schedule = SSSP_SCHED_HEURISTIC)
+        : omp_threads_(omp_threads), device_(device), schedule_(schedule) {}
+    void set_omp_threads(int n) { omp_threads_ = n; }
+    void set_schedule(int schedule) { schedule_ = schedule; }
+
+    double gpu_bfs(Graph<_TVertexValue, _TEdgeWeight> &graph, bool *result, int source_vertex)
+    {
+        static_assert(sizeof(bool) == 1, "bool must be one byte, as the visited array");
+        sssp_graph *g = cache_.get(graph, device_, NULL);
+        sssp_stats st;
+        sssp_host::check(sssp_bfs(g, source_vertex, schedule_, reinterpret_cast<uint8_t *>(result),
+                                  NULL, &st));
+        return graph.get_edges_count() / (st.t_solve_ms * 1e-3 * 1e6);
+    }
+
+  private:
+    int omp_threads_;
+    int device_;
+    int schedule_;
+    sssp_host::DeviceGraphCache<_TVertexValue, _TEdgeWeight> cache_;
 };

@@ -88,6 +88,8 @@ _SIGS = {
     "sssp_init_distances_f64": (I32, [P, I32, I32]),
     "sssp_init_distances_u32": (I32, [P, I32, I32]),
     "sssp_bellman_ford_f32": (I32, [P, P, P, I32, I64, I32, P, PI32]),
+    "sssp_bfs": (I32, [P, I32, I32, P, P, P]),
+    "sssp_bfs_edges": (I32, [P, P, I32, I64, P, I32, PI32]),
     "sssp_bellman_ford_f64": (I32, [P, P, P, I32, I64, I32, P, PI32]),
     "sssp_bellman_ford_u32": (I32, [P, P, P, I32, I64, I32, P, PI32]),
     "sssp_rmat_generate": (I32, [I32, I32, I32, I32, I32, I32, U32, I32, P, P, P]),
@@ -328,6 +330,17 @@ class DeviceGraph:
         _check(lib().sssp_solve(self._h, int(source), sched, _ptr(d), C.byref(st)))
         return d, st.as_dict()
 
+    def bfs(self, source: int, schedule="heuristic", levels=True):
+        """BFS from `source` on the same engine (weights ignored): returns the
+        reference's visited flags (bool), the hop levels (uint32, 0xFFFFFFFF
+        unreached) when asked, and the solve stats."""
+        sched = SCHED_NAMES[schedule] if isinstance(schedule, str) else int(schedule)
+        st = Stats()
+        vis = np.empty(self.V, np.uint8)
+        lev = np.empty(self.V, np.uint32) if levels else None
+        _check(lib().sssp_bfs(self._h, int(source), sched, _ptr(vis), _ptr(lev), C.byref(st)))
+        return vis.view(np.bool_), lev, st.as_dict()
+
     def distances(self):
         d = np.empty(self.V, self.wdtype)
         _check(lib().sssp_get_distances(self._h, _ptr(d)))
@@ -549,6 +562,25 @@ class GraphStorageAPI:
             raise SSSPError(ERR_ARG, "loading directed graph to non-directed container"
                                      "(or undirected to directed)")
         return EdgesListGraph(V, s, t, w, d)
+
+
+class BFS:
+    """Drop-in for BFS<int, W>::gpu_bfs (algorithm/bfs/bfs.hpp:213-253): fills
+    the bool `result` with reachability from the source and returns MTEPS
+    over the solve window. The device graph is cached per graph object."""
+
+    def __init__(self, omp_threads=1, device=0, schedule="heuristic"):
+        self.omp_threads = omp_threads
+        self._sssp = SingleSourceShortestPaths(omp_threads, device, schedule)
+
+    def set_omp_threads(self, n):
+        self.omp_threads = n
+
+    def gpu_bfs(self, graph: EdgesListGraph, result: np.ndarray, source_vertex: int):
+        g = self._sssp._device_graph(graph)
+        vis, _, st = g.bfs(source_vertex, self._sssp.schedule, levels=False)
+        result[:] = vis
+        return graph.get_edges_count() / (st["t_solve_ms"] * 1e-3 * 1e6)
 
 
 class SingleSourceShortestPaths:

@@ -42,7 +42,19 @@ class Oracle:
             getattr(L, n).restype = C.c_int
         L.orc_max_threads.restype = C.c_int
         L.orc_wtime.restype = C.c_double
+        L.orc_bfs.argtypes = [p, p, i32, i64, i32, p, p]
+        L.orc_bfs.restype = C.c_int64
         self.L = L
+
+    def bfs(self, src, dst, V, source):
+        """(visited uint8, level uint32, reached count)"""
+        vis = np.empty(V, np.uint8)
+        lev = np.empty(V, np.uint32)
+        n = self.L.orc_bfs(_ptr(np.ascontiguousarray(src, np.int32)),
+                           _ptr(np.ascontiguousarray(dst, np.int32)), V, len(src), source,
+                           _ptr(vis), _ptr(lev))
+        assert n >= 0
+        return vis, lev, n
 
     def bf(self, src, dst, w, V, source, threads=0):
         w = np.ascontiguousarray(w)
@@ -95,6 +107,7 @@ def ref_lib():
     L.ref_cpu_bellman_ford_u32.argtypes = [p, p, p, i32, i64, i32, i32, p]
     L.ref_save_bin_f32.argtypes = [C.c_char_p, p, p, p, i32, i64]
     L.ref_load_bin_f32.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i64), p, p, p]
+    L.ref_cpu_sequential_bfs.argtypes = [p, p, i32, i64, i32, p]
     return L
 
 

@@ -20,6 +20,14 @@ static void write_f32(const std::string &path, const float *d, int n)
     std::fclose(f);
 }
 
+static void write_bytes(const std::string &path, const bool *d, int n)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) throw "cannot write output";
+    std::fwrite(d, 1, (size_t)n, f);
+    std::fclose(f);
+}
+
 int main(int argc, char **argv)
 {
     try {
@@ -88,6 +96,19 @@ int main(int argc, char **argv)
         bellman_ford_wrapper<float>(ds, dd, dw, V, E, 5, dist);
         if (hipMemcpy(result.data(), dist, V * 4, hipMemcpyDeviceToHost)) throw "hipMemcpy failed";
         write_f32(dir + "/wrap_5.f32", result.data(), V);
+        // BFS on the same engine (algorithm/bfs/bfs.hpp:213-253) and the
+        // device-array wrapper (bfs.cu:44-85)
+        BFS<int, float> bfs;
+        bool *reach = new bool[V];
+        bfs.gpu_bfs(graph, reach, 5);
+        write_bytes(dir + "/bfs_5.u8", reach, V);
+        bool *dvis = NULL;
+        if (hipMalloc((void **)&dvis, V)) throw "hipMalloc failed";
+        bfs_wrapper(ds, dd, V, E, dvis, 77);
+        if (hipMemcpy(reach, dvis, V, hipMemcpyDeviceToHost)) throw "hipMemcpy failed";
+        write_bytes(dir + "/bfsw_77.u8", reach, V);
+        delete[] reach;
+        (void)hipFree(dvis);
         (void)hipFree(ds);
         (void)hipFree(dd);
         (void)hipFree(dw);

@@ -3,7 +3,8 @@
 Runs in the build container only: it loads oracle/_ref/libref_sssp.so, which
 `make -C oracle ref` compiles from the reference sources in /root/reference
 (GraphGenerationAPI::R_MAT_parallel and random_uniform,
-SingleSourceShortestPaths::cpu_bellman_ford, GraphStorageAPI edge-list I/O).
+SingleSourceShortestPaths::cpu_bellman_ford, BFS::cpu_sequential_bfs,
+GraphStorageAPI edge-list I/O).
 The outputs are data only (inputs and expected outputs).
 
     python tests/golden/make_golden.py
@@ -40,6 +41,7 @@ def main():
     L.ref_cpu_bellman_ford_f32.argtypes = [p, p, p, i32, i64, i32, i32, p]
     L.ref_cpu_bellman_ford_u32.argtypes = [p, p, p, i32, i64, i32, i32, p]
     L.ref_save_bin_f32.argtypes = [C.c_char_p, p, p, p, i32, i64]
+    L.ref_cpu_sequential_bfs.argtypes = [p, p, i32, i64, i32, p]
 
     def rmat(V, ef):
         E = V * ef
@@ -67,6 +69,7 @@ def main():
     #    one generator thread (seed 0), source 0 -- hashes plus samples
     V, ef = 1 << 16, 16
     s, d, w = rmat(V, ef)
+    s16, d16 = s, d
     dist = bf(s, d, w, V, 0, threads=8)
     ei = np.sort(rng.choice(len(s), 4096, replace=False)).astype(np.int64)
     vi = np.sort(rng.choice(V, 4096, replace=False)).astype(np.int64)
@@ -106,6 +109,24 @@ def main():
     bw = np.array([0.5, 0.25, 1.0], np.float32)
     path = os.path.join(HERE, "three_edges.bin")
     assert L.ref_save_bin_f32(path.encode(), P(bs), P(bd), P(bw), 3, 3) == 0
+
+    # 6. BFS::cpu_sequential_bfs reachability (bfs.hpp:3-45) on the tiny R-MAT
+    #    graphs of section 1 (same sources) and on the scale-16 graph
+    def bfs(s, d, V, src):
+        out = np.empty(V, np.uint8)
+        assert L.ref_cpu_sequential_bfs(P(s), P(d), V, len(s), src, P(out)) == 0
+        return out
+
+    vis = {}
+    for V, ef in [(256, 4), (1024, 8)]:
+        s1, d1, _ = rmat(V, ef)
+        srcs = np.array([0, 1, int(s1[len(s1) // 2])], np.int32)
+        vis[f"v{V}_ef{ef}"] = np.stack([bfs(s1, d1, V, int(x)) for x in srcs])
+    np.savez(os.path.join(HERE, "bfs_ref.npz"), **vis)
+    V = 1 << 16
+    vb = bfs(s16, d16, V, 0)
+    meta["rmat_s16_ef16"]["bfs_sha256_visited_source0"] = sha(vb)
+    meta["rmat_s16_ef16"]["bfs_reached_source0"] = int(vb.sum())
 
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)

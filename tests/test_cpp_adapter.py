@@ -1,6 +1,6 @@
 """The C++ host mirror (parallel-task1_amd/host/graph_library.h) compiles with
 driver code in the shape of the reference's sample/sssp.cpp and runs it:
-containers, generator and storage on the CPU; gpu_bellman_ford and the
+containers, generator and storage on the CPU; gpu_bellman_ford, gpu_bfs and the
 reference-shaped device wrappers on the GPU, checked against the oracle."""
 import os
 import subprocess
@@ -47,3 +47,8 @@ def test_driver_gpu_path_matches_oracle(gpu, oracle, driver, tmp_path):
         assert np.array_equal(bits(got), bits(oracle.dijkstra(s, t, w, V, src)))
     got = np.fromfile(tmp_path / "wrap_5.f32", np.float32)
     assert np.array_equal(bits(got), bits(oracle.dijkstra(s, t, w, V, 5)))
+    # BFS<int,float>::gpu_bfs from 5 and bfs_wrapper from 77 against the oracle
+    for name, src in (("bfs_5.u8", 5), ("bfsw_77.u8", 77)):
+        got = np.fromfile(tmp_path / name, np.uint8)
+        vis, _, _ = oracle.bfs(s, t, V, src)
+        assert np.array_equal(got, vis), name

@@ -96,6 +96,17 @@ def test_c2_rmat22_ef32_sixteen_sources(gpu, oracle):
         assert np.array_equal(bits(d), bits(dj))
         dc, passes, sec = _cpu_bf(oracle, src, dst, w, V, int(sources[0]))
         assert np.array_equal(bits(dc), bits(d))
+        # BFS on the same graph and engine, every schedule against the oracle
+        vref, lref, nref = oracle.bfs(src, dst, V, int(sources[0]))
+        tb = {}
+        for sched in ("learned", "heuristic", "plain"):
+            g.bfs(int(sources[0]), sched, levels=False)
+            vis, lev, stb = g.bfs(int(sources[0]), sched)
+            assert np.array_equal(lev, lref) and np.array_equal(vis.view(np.uint8), vref), sched
+            tb[sched] = round(_gteps(E, stb["t_solve_ms"]), 2)
+        ok, counts, _ = g.certify(int(sources[0]), parents=False)
+        assert ok, counts
+        print("C2 BFS", tb, "GTEPS,", nref, "reached")
         print("C2", {k: round(_gteps(E * len(v), sum(v)), 2) for k, v in t.items()}, "GTEPS; CPU",
               round(E / sec / 1e9, 3), "GTEPS (1 source,", passes, "passes)")
         del m

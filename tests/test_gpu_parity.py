@@ -319,3 +319,51 @@ def test_far_pile_after_failed_solve_and_collection(gpu, oracle):
         assert len(feats) > 0 and np.all(np.isfinite(times[:, :8]))
         d, _ = g.solve(3, "heuristic")
         assert np.array_equal(bits(d), bits(ref))
+
+
+# ---- BFS on the same engine (BFS::gpu_bfs, algorithm/bfs/bfs.hpp:213-253) -----
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,V,src,dst,w", small_graphs(), ids=[g[0] for g in small_graphs()])
+def test_bfs_small_graphs_every_schedule(gpu, oracle, name, V, src, dst, w):
+    vis_ref, lev_ref, n = oracle.bfs(src, dst, V, 0)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        for sched in SCHEDS:
+            vis, lev, st = g.bfs(0, sched)
+            assert np.array_equal(vis.view(np.uint8), vis_ref), sched
+            assert np.array_equal(lev, lev_ref), sched
+            assert st["reached"] == n
+        ok, counts, _ = g.certify(0)
+        assert ok, counts
+
+
+@pytest.mark.gpu
+def test_bfs_rmat_and_er_every_schedule(gpu, oracle):
+    src, dst, w = S.rmat(14, 16, seed=17, threads=16)
+    V = 1 << 14
+    with S.DeviceGraph(src, dst, w, V) as g:
+        for s in [0] + list(S.pick_sources(src, V, 5, 3)):
+            vis_ref, lev_ref, _ = oracle.bfs(src, dst, V, int(s))
+            for sched in SCHEDS:
+                vis, lev, _ = g.bfs(int(s), sched)
+                assert np.array_equal(lev, lev_ref), (int(s), sched)
+                assert np.array_equal(vis.view(np.uint8), vis_ref)
+            ok, counts, par = g.certify(int(s))
+            assert ok, counts
+            # the tight tree of a BFS: every visited non-source vertex's parent
+            # is one level up
+            m = (par >= 0)
+            assert np.all(lev[m] == lev[par[m]] + 1)
+            # BFS levels are not distances: the SSSP result is gone
+            with pytest.raises(S.SSSPError):
+                g.distances()
+        d, _ = g.solve(3, "heuristic")
+        assert np.array_equal(bits(d), bits(oracle.dijkstra(src, dst, w, V, 3)))
+        ok, counts, _ = g.certify(3)
+        assert ok, counts
+    su, tu, wu = S.er(1 << 13, 8, seed=11, weight_mode=1)
+    with S.DeviceGraph(su, tu, wu, 1 << 13) as g:
+        vis_ref, lev_ref, _ = oracle.bfs(su, tu, 1 << 13, 0)
+        for sched in ("plain", "heuristic", "push", "pull"):
+            vis, lev, _ = g.bfs(0, sched)
+            assert np.array_equal(lev, lev_ref), sched

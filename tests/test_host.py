@@ -49,6 +49,8 @@ def test_argument_errors_are_status_codes(lib):
     assert lib.sssp_set_param(None, b"verify", 1.0) == S.ERR_ARG
     assert lib.sssp_init_distances_f32(None, 10, 0) == S.ERR_ARG
     assert lib.sssp_bellman_ford_f32(None, None, None, 10, 5, 0, None, None) == S.ERR_ARG
+    assert lib.sssp_bfs_edges(None, None, 10, 5, None, 0, None) == S.ERR_ARG
+    assert lib.sssp_bfs(None, 0, 0, None, None, None) == S.ERR_ARG
     with pytest.raises(S.SSSPError):
         S.rmat(0, 4)
     with pytest.raises(S.SSSPError):

@@ -262,3 +262,63 @@ def test_certificate_checks_parents(oracle):
     assert oracle.certify(src, dst, w, 4, 0, dist, bad)[0] != 0
     bad = np.array([-1, 2, 1, 2], np.int32)    # 1 <- 2 is not an edge
     assert oracle.certify(src, dst, w, 4, 0, dist, bad)[0] != 0
+
+
+# ---- BFS (BFS::cpu_sequential_bfs, algorithm/bfs/bfs.hpp:3-45) ----------------
+
+@pytest.mark.parametrize("V,ef", [(256, 4), (1024, 8)])
+def test_oracle_bfs_matches_reference_vectors(oracle, V, ef):
+    g = gold(f"rmat_v{V}_ef{ef}.npz")
+    vis = gold("bfs_ref.npz")[f"v{V}_ef{ef}"]
+    for i, s in enumerate(g["sources"]):
+        v, lev, n = oracle.bfs(g["src"], g["dst"], V, int(s))
+        assert np.array_equal(v, vis[i]) and n == int(vis[i].sum())
+
+
+def test_oracle_bfs_c1_matches_reference_hash(oracle, meta):
+    s, d, _ = S.rmat(16, 16, seed=1, threads=1)
+    v, lev, n = oracle.bfs(s, d, 1 << 16, 0)
+    m = meta["rmat_s16_ef16"]
+    assert sha(v) == m["bfs_sha256_visited_source0"]
+    assert n == m["bfs_reached_source0"]
+
+
+def test_oracle_bfs_against_compiled_reference(oracle, ref_lib):
+    _p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    for seed in range(4):
+        s, d, _ = S.er(300, 3, seed=seed + 40, weight_mode=0)
+        for src in (0, 17, 299):
+            ref = np.empty(300, np.uint8)
+            assert ref_lib.ref_cpu_sequential_bfs(_p(s), _p(d), 300, len(s), src, _p(ref)) == 0
+            v, _, _ = oracle.bfs(s, d, 300, src)
+            assert np.array_equal(v, ref)
+
+
+def test_oracle_bfs_levels_are_unit_weight_distances(oracle):
+    """A FIFO BFS's discovery level is the hop distance: it equals Dijkstra
+    with every weight 1 (an independent algorithm), and visited == finite."""
+    for seed in range(3):
+        s, d, _ = S.rmat(11, 8, seed=seed + 5, threads=4)
+        V = 1 << 11
+        one = np.ones(len(s), np.uint32)
+        for src in (0, int(s[7])):
+            v, lev, n = oracle.bfs(s, d, V, src)
+            dj = oracle.dijkstra(s, d, one, V, src)
+            assert np.array_equal(lev, dj)
+            assert np.array_equal(v.astype(bool), dj != 0xFFFFFFFF) and n == int(v.sum())
+
+
+@pytest.mark.parametrize("name,V,src,dst,w", small_graphs(), ids=[g[0] for g in small_graphs()])
+def test_oracle_bfs_closed_forms(oracle, name, V, src, dst, w):
+    v, lev, n = oracle.bfs(src, dst, V, 0)
+    assert v[0] == 1 and lev[0] == 0
+    if name == "chain":
+        assert np.array_equal(lev, np.arange(V, dtype=np.uint32)) and n == V
+    if name == "star":
+        assert np.all(lev[1:] == 1) and n == V
+    if name == "disconnected":
+        assert v.tolist() == [1, 1, 1, 0, 0, 0] and lev[2] == 2 and lev[3] == 0xFFFFFFFF
+    if name == "loops_dups":
+        assert lev.tolist() == [0, 1, 2, 3]
+    if name == "empty":
+        assert n == 1
