@@ -159,7 +159,10 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
                  int32_t *valid);
 /* Tuning knobs by name ("push_chunk", "all_chunk", "pull_chunk",
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
- * "delta", "hub_degree", "trace_events"; "cu_count" read-only).
+ * "delta", "hub_degree", "plain_below_edges", "trace_events"; "cu_count"
+ * read-only). plain_below_edges (default 2^25): a single-source heuristic or
+ * learned solve (or BFS) of a graph with fewer edges runs the plain pass loop,
+ * whose passes then cost less than a round's fixed overhead.
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with
  * d >= (min distance over the frontier and pile) + delta and out-degree >=
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.

@@ -1405,7 +1405,13 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
             g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, g->src_list, n_src, tick0);
         SSSP_LAUNCH_CHECK("k_init_sources");
     }
-    if (sched == SSSP_SCHED_PLAIN) {
+    // Below plain_below_edges the rule and the network hand the whole solve
+    // to the plain pass loop: a pass then costs less than a round's fixed
+    // host read-back and small-kernel launches (measured crossover between
+    // R-MAT scale 20 and 22 at edge factor 16).
+    const bool small = (sched == SSSP_SCHED_HEURISTIC || sched == SSSP_SCHED_LEARNED) &&
+                       n_src == 1 && g->E < g->plain_below_edges;
+    if (sched == SSSP_SCHED_PLAIN || small) {
         solve_plain<WT>(g, st);
         g->tick = tick0 + 1;
     } else {

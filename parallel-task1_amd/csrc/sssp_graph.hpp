@@ -142,6 +142,7 @@ struct sssp_graph {
     double pull_frac = 0.5;        // heuristic: pull when unsettled in-edges < pull_frac * frontier edges
     double delta = 0.0;            // push rounds defer vertices with d >= min + delta (0 = off)
     int hub_degree = 0;            // ... unless their out-degree is below this
+    int64_t plain_below_edges = (int64_t)1 << 25;  // heuristic/learned: plain passes below this E
     int trace_events = 0;
 
     // last solve

@@ -78,6 +78,7 @@ def test_any_network_keeps_the_fixpoint(gpu, oracle):
     V = 1 << 13
     ref = oracle.dijkstra(src, dst, w, V, 0)
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         for seed in range(3):
             m = S.MLP(0, seed=seed)
             g.set_mlp(m)

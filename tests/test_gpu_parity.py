@@ -34,6 +34,7 @@ def test_device_is_gfx950(gpu):
 @pytest.mark.parametrize("name,V,src,dst,w", small_graphs(), ids=[g[0] for g in small_graphs()])
 def test_small_graphs_every_schedule(gpu, oracle, name, V, src, dst, w):
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         for sched in SCHEDS:
             d, st = g.solve(0, sched)
             _check_solution(g, oracle, src, dst, w, V, 0, d, sched)
@@ -52,6 +53,8 @@ def test_device_ingest_from_bin(gpu, oracle, tmp_path):
     p = tmp_path / "g.bin"
     S.save_bin(p, src, dst, w, V)
     with S.DeviceGraph.from_bin(p) as g1, S.DeviceGraph(src, dst, w, V) as g2:
+        g1.set_param("plain_below_edges", 0)  # exercise the round engine
+        g2.set_param("plain_below_edges", 0)  # exercise the round engine
         assert (g1.V, g1.E) == (V, len(src))
         L1, L2 = g1.layout(), g2.layout()
         for k in L1:
@@ -64,6 +67,7 @@ def test_device_ingest_from_bin(gpu, oracle, tmp_path):
     pu = tmp_path / "u.bin"
     S.save_bin(pu, su, tu, wu, 1 << 12)
     with S.DeviceGraph.from_bin(pu, weight_dtype=np.uint32) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         d, _ = g.solve(0, "heuristic")
         assert np.array_equal(d, oracle.dijkstra(su, tu, wu, 1 << 12, 0))
     # several chunks (4 Mi records each) through the two pinned buffers
@@ -71,6 +75,8 @@ def test_device_ingest_from_bin(gpu, oracle, tmp_path):
     p2 = tmp_path / "big.bin"
     S.save_bin(p2, s2, t2, w2, 1 << 20)
     with S.DeviceGraph.from_bin(p2, csc=False) as g1, S.DeviceGraph(s2, t2, w2, 1 << 20, csc=False) as g2:
+        g1.set_param("plain_below_edges", 0)  # exercise the round engine
+        g2.set_param("plain_below_edges", 0)  # exercise the round engine
         a, _ = g1.solve(5, "plain")
         b, _ = g2.solve(5, "plain")
         assert np.array_equal(bits(a), bits(b))
@@ -83,6 +89,7 @@ def test_layout_is_stable_sort(gpu):
     src, dst, w = S.rmat(11, 16, seed=3, threads=4)
     V = 1 << 11
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         L = g.layout()
     p = np.argsort(src, kind="stable")
     assert np.array_equal(L["el_src"], src[p])
@@ -104,6 +111,7 @@ def test_rmat_every_schedule_bit_exact(gpu, oracle, scale, ef, seed, threads):
     V = 1 << scale
     sources = [0] + list(S.pick_sources(src, V, seed + 100, 3))
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         for s in sources:
             ref, _ = oracle.bf(src, dst, w, V, s)
             for sched in SCHEDS:
@@ -122,6 +130,7 @@ def test_er_u32_bit_exact(gpu, oracle, logV, deg):
     V = 1 << logV
     src, dst, w = S.er(V, deg, seed=3, weight_mode=1)
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         ref, _ = oracle.bf(src, dst, w, V, 0)
         dj = oracle.dijkstra(src, dst, w, V, 0)
         assert np.array_equal(ref, dj)
@@ -142,6 +151,8 @@ def test_edge_order_does_not_change_distances(gpu):
     V = 1 << 13
     perm = np.random.default_rng(0).permutation(len(src))
     with S.DeviceGraph(src, dst, w, V) as g1, S.DeviceGraph(src[perm], dst[perm], w[perm], V) as g2:
+        g1.set_param("plain_below_edges", 0)  # exercise the round engine
+        g2.set_param("plain_below_edges", 0)  # exercise the round engine
         for sched in ("plain", "heuristic"):
             a, _ = g1.solve(5, sched)
             b, _ = g2.solve(5, sched)
@@ -154,6 +165,7 @@ def test_repeat_solves_reuse_graph(gpu, oracle):
     src, dst, w = S.rmat(12, 8, seed=7, threads=8)
     V = 1 << 12
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         for s in [0, 1, 2, 0, 3]:
             for sched in ("push", "all", "pull", "heuristic"):
                 d, _ = g.solve(s, sched)
@@ -165,6 +177,7 @@ def test_repeat_solves_reuse_graph(gpu, oracle):
 def test_verify_pass_and_trace(gpu):
     src, dst, w = S.rmat(12, 16, seed=2, threads=8)
     with S.DeviceGraph(src, dst, w, 1 << 12) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         g.set_param("trace_events", 1)
         d, st = g.solve(0, "heuristic")
         tr = g.trace()
@@ -186,6 +199,7 @@ def test_device_rmat_generator_matches_host(gpu, scale, ef, seed, threads):
     finally:
         g.close()
     with S.DeviceGraph(s, d, w, 1 << scale) as h:
+        h.set_param("plain_below_edges", 0)  # exercise the round engine
         M = h.layout()
     for k in ("row_ptr", "el_src", "el_dst", "col_ptr", "csc_src"):
         assert np.array_equal(L[k], M[k]), k
@@ -199,6 +213,7 @@ def test_batch_sources_match_single_solves(gpu, oracle, policy):
     V = 1 << 13
     sources = S.pick_sources(src, V, 21, 48)
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         D, st = g.solve_batch(sources, policy=policy)
         assert D.shape == (48, V) and st["t_solve_ms"] > 0
         for i in (0, 7, 47):
@@ -218,6 +233,7 @@ def test_batch_u32(gpu, oracle):
     src, dst, w = S.er(1 << 12, 8, seed=4, weight_mode=1)
     V = 1 << 12
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         D, _ = g.solve_batch(np.array([0, 5, 9], np.int32), policy=1)
         for i, s in enumerate((0, 5, 9)):
             assert np.array_equal(D[i], oracle.dijkstra(src, dst, w, V, s))
@@ -254,6 +270,7 @@ def test_autotune_batch_and_pack(gpu, tmp_path):
     V = 1 << 12
     sources = S.pick_sources(src, V, 2, 16)
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         ref, _ = g.solve_batch(sources, policy=2)
         best = g.autotune_batch(sources, tmp_path / "tune.txt")
         assert best > 0
@@ -281,6 +298,7 @@ def test_far_pile_keeps_the_fixpoint(gpu, oracle, delta, hub):
     src, dst, w = S.rmat(14, 16, seed=31, threads=16)
     V = 1 << 14
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         g.set_param("delta", delta)
         g.set_param("hub_degree", hub)
         for s in (0, int(src[123])):
@@ -293,6 +311,7 @@ def test_far_pile_keeps_the_fixpoint(gpu, oracle, delta, hub):
         assert ok, counts
     su, tu, wu = S.er(1 << 13, 8, seed=9, weight_mode=1)
     with S.DeviceGraph(su, tu, wu, 1 << 13) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         g.set_param("delta", delta * 100)
         g.set_param("hub_degree", hub)
         d, _ = g.solve(0, "push")
@@ -307,6 +326,7 @@ def test_far_pile_after_failed_solve_and_collection(gpu, oracle):
     V = 1 << 13
     ref = oracle.dijkstra(src, dst, w, V, 3)
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         g.set_param("delta", 0.01)
         g.set_param("hub_degree", 32)
         g.set_param("max_rounds", 4)
@@ -328,6 +348,7 @@ def test_far_pile_after_failed_solve_and_collection(gpu, oracle):
 def test_bfs_small_graphs_every_schedule(gpu, oracle, name, V, src, dst, w):
     vis_ref, lev_ref, n = oracle.bfs(src, dst, V, 0)
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         for sched in SCHEDS:
             vis, lev, st = g.bfs(0, sched)
             assert np.array_equal(vis.view(np.uint8), vis_ref), sched
@@ -342,6 +363,7 @@ def test_bfs_rmat_and_er_every_schedule(gpu, oracle):
     src, dst, w = S.rmat(14, 16, seed=17, threads=16)
     V = 1 << 14
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         for s in [0] + list(S.pick_sources(src, V, 5, 3)):
             vis_ref, lev_ref, _ = oracle.bfs(src, dst, V, int(s))
             for sched in SCHEDS:
@@ -363,7 +385,29 @@ def test_bfs_rmat_and_er_every_schedule(gpu, oracle):
         assert ok, counts
     su, tu, wu = S.er(1 << 13, 8, seed=11, weight_mode=1)
     with S.DeviceGraph(su, tu, wu, 1 << 13) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
         vis_ref, lev_ref, _ = oracle.bfs(su, tu, 1 << 13, 0)
         for sched in ("plain", "heuristic", "push", "pull"):
             vis, lev, _ = g.bfs(0, sched)
             assert np.array_equal(lev, lev_ref), sched
+
+
+@pytest.mark.gpu
+def test_small_graphs_hand_the_rule_and_network_to_plain_passes(gpu, oracle):
+    """Below plain_below_edges a heuristic or learned solve (and BFS) runs the
+    plain pass loop -- same fixpoint, no rounds; 0 switches this off."""
+    src, dst, w = S.rmat(13, 16, seed=8, threads=8)
+    V = 1 << 13
+    ref = oracle.dijkstra(src, dst, w, V, 0)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        assert g.get_param("plain_below_edges") == 2 ** 25
+        for sched in ("heuristic", "learned"):
+            d, st = g.solve(0, sched)
+            assert np.array_equal(bits(d), bits(ref)), sched
+            assert st["n_push"] == 0 and st["n_pull"] == 0 and st["mlp_rounds"] == 0
+            assert st["n_all"] == st["iterations"] > 0
+        vis, lev, st = g.bfs(0, "heuristic")
+        assert np.array_equal(lev, oracle.bfs(src, dst, V, 0)[1]) and st["n_push"] == 0
+        g.set_param("plain_below_edges", 0)
+        d, st = g.solve(0, "heuristic")
+        assert np.array_equal(bits(d), bits(ref)) and st["n_push"] > 0
