@@ -159,8 +159,9 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
                  int32_t *valid);
 /* Tuning knobs by name ("push_chunk", "all_chunk", "pull_chunk",
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
- * "delta", "hub_degree", "plain_below_edges", "trace_events"; "cu_count"
- * read-only). plain_below_edges (default 2^25): a single-source heuristic or
+ * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "trace_events";
+ * "cu_count" read-only). rollout_policy (sssp_collect_timings): 0 = the rounds
+ * after the tried one follow the rule, 1 = the network set on the graph. plain_below_edges (default 2^25): a single-source heuristic or
  * learned solve (or BFS) of a graph with fewer edges runs the plain pass loop,
  * whose passes then cost less than a round's fixed overhead.
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with

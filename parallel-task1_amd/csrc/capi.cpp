@@ -509,6 +509,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         else if (n == "delta") {
             SSSP_REQUIRE(value >= 0, "delta must be >= 0");
             g->delta = value;
+        } else if (n == "rollout_policy") {
+            SSSP_REQUIRE(value == 0 || value == 1, "rollout_policy must be 0 (rule) or 1 (network)");
+            g->rollout_policy = (int)value;
         } else if (n == "plain_below_edges") {
             SSSP_REQUIRE(value >= 0 && value <= 9.2e18, "plain_below_edges must be >= 0");
             g->plain_below_edges = (int64_t)value;
@@ -536,6 +539,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "delta") *value = g->delta;
         else if (n == "hub_degree") *value = g->hub_degree;
         else if (n == "plain_below_edges") *value = (double)g->plain_below_edges;
+        else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "trace_events") *value = g->trace_events;
         else if (n == "cu_count") *value = g->cu_count;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};

@@ -1496,7 +1496,8 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
 // Timing collection with rollouts. At every round r of a solve the state is
 // saved (distances, stamps, pile states, the round's queue and pile, the
 // control block) and each mode x chunk bucket c is tried: round r runs c and
-// the rounds after it follow the hand-written rule to the end of the solve;
+// the rounds after it follow the hand-written rule (or, rollout_policy = 1,
+// the network set on the graph) to the end of the solve;
 // times[r][c] is the best of two device timings of that whole remainder (host
 // round trips included, as in a real solve). The solve then advances by the best c. The
 // labels are thus one step of policy improvement over the rule: a round that
@@ -1546,6 +1547,9 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
     k_init_solve<<<1, 64, 0, s>>>(g->ctrl, g->dist, g->stamp, g->queue[0], g->row_ptr, source, tick0);
     SSSP_LAUNCH_CHECK("collect init");
     const int nt = kModes * kChunkCount;
+    // the policy the rounds after the tried one follow: the rule, or (one more
+    // step of policy improvement) the network set on the graph
+    const int cont = (g->rollout_policy == 1 && g->mlp) ? SSSP_SCHED_LEARNED : SSSP_SCHED_HEURISTIC;
     int32_t row = 0, r = 0, r_hi = 0;
     bool verified = false;
     int64_t relaxed = 0;
@@ -1609,7 +1613,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
                                 throw Error{SSSP_ERR_NOT_CONVERGED,
                                             "collect: rollout did not converge"};
                             }
-                            if (!step_round<WT>(g, r2, SSSP_SCHED_HEURISTIC, tick0, &tmp, ver, rel))
+                            if (!step_round<WT>(g, r2, cont, tick0, &tmp, ver, rel))
                                 break;
                         }
                         r_hi = r2 > r_hi ? r2 : r_hi;

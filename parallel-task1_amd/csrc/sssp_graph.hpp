@@ -143,6 +143,7 @@ struct sssp_graph {
     double delta = 0.0;            // push rounds defer vertices with d >= min + delta (0 = off)
     int hub_degree = 0;            // ... unless their out-degree is below this
     int64_t plain_below_edges = (int64_t)1 << 25;  // heuristic/learned: plain passes below this E
+    int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int trace_events = 0;
 
     // last solve

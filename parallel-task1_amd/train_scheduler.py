@@ -9,7 +9,9 @@ solve follows the hand-written rule; the device time of the whole remainder is
 the choice's cost (a rollout). The solve then advances by the best choice.
 Each round becomes one training row: the 5 round features -> the mode with the
 lowest remainder time and, for that mode, the best chunk bucket. A network
-trained on these labels is one step of policy improvement over the rule. It
+trained on these labels is one step of policy improvement over the rule;
+--rollout-weights makes the rollouts continue with a trained network instead,
+the next step of policy iteration. It
 (5 -> 64 -> 64 -> 3 + 4) is trained on the GPU by the library's own
 forward/backward/Adam kernels, and the weights are written to
 models/scheduler_mlp.f32 (raw float32, the library's parameter order) with a
@@ -33,11 +35,20 @@ import sssp_amd as S  # noqa: E402
 MODELS = os.path.join(HERE, "models")
 
 
-def collect(scales, sources_per_scale, edge_factor, seed0):
+def _rollout_with(g, net):
+    """Rollouts continue with `net` (one more policy-improvement step) when
+    given, else with the hand-written rule."""
+    if net is not None:
+        g.set_mlp(net)
+        g.set_param("rollout_policy", 1)
+
+
+def collect(scales, sources_per_scale, edge_factor, seed0, net=None):
     X, T, meta = [], [], []
     for k, scale in enumerate(scales):
         seed = seed0 + k
         g = S.DeviceGraph.rmat(scale, edge_factor, seed=seed, threads=64)
+        _rollout_with(g, net)
         L = g.layout(csc=False)
         deg = np.diff(L["row_ptr"])
         srcs = S.pick_sources(np.repeat(np.arange(g.V, dtype=np.int32), deg), g.V, seed,
@@ -55,7 +66,7 @@ def collect(scales, sources_per_scale, edge_factor, seed0):
     return np.concatenate(X), np.concatenate(T), meta
 
 
-def collect_er(logvs, sources_per_graph, degree, seed0):
+def collect_er(logvs, sources_per_graph, degree, seed0, net=None):
     """Uniform random graphs (random_uniform, integer weights 1..255)."""
     X, T, meta = [], [], []
     for k, logv in enumerate(logvs):
@@ -63,6 +74,7 @@ def collect_er(logvs, sources_per_graph, degree, seed0):
         V = 1 << logv
         src, dst, w = S.er(V, degree, seed=seed, weight_mode=1)
         g = S.DeviceGraph(src, dst, w, V)
+        _rollout_with(g, net)
         srcs = S.pick_sources(src, V, seed, sources_per_graph)
         del src, dst, w
         for s in srcs:
@@ -122,6 +134,9 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=os.path.join(MODELS, "scheduler_mlp.f32"))
     ap.add_argument("--data", help="train on the X/T arrays of an earlier run's _data.npz")
+    ap.add_argument("--rollout-weights",
+                    help="weights of a network the rollouts continue with (policy iteration); "
+                         "default: the hand-written rule")
     ap.add_argument("--min-spread", type=float, default=1.02)
     ap.add_argument("--max-copies", type=int, default=8)
     args = ap.parse_args()
@@ -131,12 +146,17 @@ def main():
             X, T = z["X"], z["T"]
         meta = [{"data": os.path.basename(args.data)}]
     else:
-        X, T, meta = collect(args.scales, args.sources, args.edge_factor, args.seed)
+        net = None
+        if args.rollout_weights:
+            net = S.MLP(0)
+            net.set_params(np.fromfile(args.rollout_weights, np.float32))
+        X, T, meta = collect(args.scales, args.sources, args.edge_factor, args.seed, net)
         if args.ef32_scales:
-            Xr, Tr, mr = collect(args.ef32_scales, args.sources, 32, args.seed + 50)
+            Xr, Tr, mr = collect(args.ef32_scales, args.sources, 32, args.seed + 50, net)
             X, T, meta = np.concatenate([X, Xr]), np.concatenate([T, Tr]), meta + mr
         if args.er_logv:
-            Xe, Te, me = collect_er(args.er_logv, args.sources, args.er_degree, args.seed + 100)
+            Xe, Te, me = collect_er(args.er_logv, args.sources, args.er_degree, args.seed + 100,
+                                    net)
             X, T, meta = np.concatenate([X, Xe]), np.concatenate([T, Te]), meta + me
     ym, yc, t = labels(T)
     n = len(X)
@@ -158,6 +178,8 @@ def main():
         "rows": int(n), "train_rows": int(len(tr)), "val_rows": int(nval),
         "train_rows_kept": int((copies > 0).sum()), "train_rows_weighted": int(len(trw)),
         "min_spread": args.min_spread, "max_copies": args.max_copies,
+        "rollout_policy": "network " + os.path.basename(args.rollout_weights)
+        if args.rollout_weights else "rule",
         "scales": args.scales, "edge_factor": args.edge_factor, "sources_per_scale": args.sources,
         "ef32_scales": args.ef32_scales, "er_logv": args.er_logv, "er_degree": args.er_degree,
         "label_mode_counts": np.bincount(ym, minlength=S.MLP_MODES).tolist(),
