@@ -183,3 +183,24 @@ def test_c4b_pack_of_8192_graphs(gpu, oracle):
                       "policy2": round(_gteps(E, st2["t_solve_ms"]), 2)}, "GTEPS", E, "edges")
     finally:
         pack.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_maximum_edge_count(gpu):
+    """The largest graph the C ABI takes (fewer than 2^31 - 1 edges): R-MAT
+    scale 26, edge factor 31 (2,080,374,784 edges), built on the device. At
+    this size the check is the certificate and schedule independence."""
+    g = S.DeviceGraph.rmat(26, 31, seed=3, threads=64, csc=False)
+    with g:
+        assert g.E == 2080374784
+        dh, sh = g.solve(7, "heuristic")
+        ok, counts, _ = g.certify(7, parents=False)
+        assert ok, counts
+        assert counts[5] == sh["reached"] == int((dh != S.F32_INF).sum())
+        dp, sp = g.solve(7, "plain")
+        assert np.array_equal(bits(dh), bits(dp))
+        print("max E", {"heuristic": round(_gteps(g.E, sh["t_solve_ms"]), 2),
+                        "plain": round(_gteps(g.E, sp["t_solve_ms"]), 2)}, "GTEPS")
+    with pytest.raises(S.SSSPError):
+        S.DeviceGraph.rmat(26, 32, seed=3, threads=64, csc=False)  # 2^31 edges: rejected
