@@ -159,11 +159,14 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
                  int32_t *valid);
 /* Tuning knobs by name ("push_chunk", "all_chunk", "pull_chunk",
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
- * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "trace_events";
- * "cu_count" read-only). rollout_policy (sssp_collect_timings): 0 = the rounds
- * after the tried one follow the rule, 1 = the network set on the graph. plain_below_edges (default 2^25): a single-source heuristic or
- * learned solve (or BFS) of a graph with fewer edges runs the plain pass loop,
- * whose passes then cost less than a round's fixed overhead.
+ * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
+ * "trace_events"; "cu_count" read-only).
+ * rollout_policy and collect_kind steer sssp_collect_timings: the rounds
+ * after the tried one follow the rule (0) or the attached network (1); the
+ * collected solve is an SSSP (0) or a BFS (1).
+ * plain_below_edges (default 2^25): a single-source heuristic or learned
+ * solve (or BFS) of a graph with fewer edges runs the plain pass loop, whose
+ * passes then cost less than a round's fixed overhead.
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with
  * d >= (min distance over the frontier and pile) + delta and out-degree >=
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.
@@ -295,11 +298,18 @@ int sssp_mlp_loss_grad(sssp_mlp *m, const float *x, const int32_t *y_mode,
 int sssp_mlp_train(sssp_mlp *m, const float *x, const int32_t *y_mode, const int32_t *y_chunk,
                    int32_t n, int32_t epochs, int32_t batch, float lr, uint64_t seed,
                    float *losses);
-/* Attach the network to a graph for SSSP_SCHED_LEARNED (NULL detaches). */
+/* Attach the network to a graph for SSSP_SCHED_LEARNED SSSP solves (NULL
+ * detaches). */
 int sssp_graph_set_mlp(sssp_graph *g, sssp_mlp *m);
-/* Timing collection for training: solves from `source`, and before every round
- * times each of the 3 modes x SSSP_MLP_CHUNKS chunk widths from the same
- * saved state (device events), then continues with the fastest. Writes up to
+/* The same for sssp_bfs with SSSP_SCHED_LEARNED; without one, a learned BFS
+ * runs the hand-written rule. */
+int sssp_graph_set_bfs_mlp(sssp_graph *g, sssp_mlp *m);
+/* Timing collection for training (rollouts): solves from `source` (a BFS when
+ * the graph's "collect_kind" is 1); before every round the state is saved and
+ * each of the 3 modes x SSSP_MLP_CHUNKS chunk widths is run for that round,
+ * the rest of the solve following the rule ("rollout_policy" 0) or the
+ * attached network (1); times_ms is the best of two device timings of each
+ * remainder. The solve then continues with the best choice. Writes up to
  * `cap` rows: features[5], times_ms[3*SSSP_MLP_CHUNKS]. */
 int sssp_collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *features,
                          float *times_ms, int32_t *rows);

@@ -509,6 +509,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         else if (n == "delta") {
             SSSP_REQUIRE(value >= 0, "delta must be >= 0");
             g->delta = value;
+        } else if (n == "collect_kind") {
+            SSSP_REQUIRE(value == 0 || value == 1, "collect_kind must be 0 (SSSP) or 1 (BFS)");
+            g->collect_kind = (int)value;
         } else if (n == "rollout_policy") {
             SSSP_REQUIRE(value == 0 || value == 1, "rollout_policy must be 0 (rule) or 1 (network)");
             g->rollout_policy = (int)value;
@@ -540,6 +543,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "hub_degree") *value = g->hub_degree;
         else if (n == "plain_below_edges") *value = (double)g->plain_below_edges;
         else if (n == "rollout_policy") *value = g->rollout_policy;
+        else if (n == "collect_kind") *value = g->collect_kind;
         else if (n == "trace_events") *value = g->trace_events;
         else if (n == "cu_count") *value = g->cu_count;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};
@@ -623,7 +627,8 @@ int sssp_collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feat
         g->bfs_result = false;
         collect_timings(g, source, cap, features, times_ms, rows);
         g->last_source = source;
-        g->have_result = true;
+        if (g->collect_kind == 1) g->bfs_result = true;  // the collected solve was a BFS
+        else g->have_result = true;
     });
 }
 

@@ -1292,7 +1292,8 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     da.push_chunk = g->push_chunk;
     da.all_chunk = g->all_chunk;
     da.pull_chunk = g->pull_chunk;
-    da.mlp = (learned && g->mlp) ? mlp_device_params(g->mlp) : nullptr;
+    sssp_mlp *net = g->solving_bfs ? g->bfs_mlp : g->mlp;
+    da.mlp = (learned && net) ? mlp_device_params(net) : nullptr;
     da.pile = g->pile_cur;
     k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
     SSSP_LAUNCH_CHECK("k_decide");
@@ -1549,7 +1550,9 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
     const int nt = kModes * kChunkCount;
     // the policy the rounds after the tried one follow: the rule, or (one more
     // step of policy improvement) the network set on the graph
-    const int cont = (g->rollout_policy == 1 && g->mlp) ? SSSP_SCHED_LEARNED : SSSP_SCHED_HEURISTIC;
+    const int cont = (g->rollout_policy == 1 && (g->solving_bfs ? g->bfs_mlp : g->mlp))
+                         ? SSSP_SCHED_LEARNED
+                         : SSSP_SCHED_HEURISTIC;
     int32_t row = 0, r = 0, r_hi = 0;
     bool verified = false;
     int64_t relaxed = 0;
@@ -1668,8 +1671,17 @@ void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp
     else solve_t<WU32>(g, sources, n_src, sched, st);
 }
 
+// Marks the rounds in flight as a BFS (its own scheduler network) for the
+// lifetime of the guard.
+struct BfsScope {
+    sssp_graph *g;
+    explicit BfsScope(sssp_graph *gg) : g(gg) { g->solving_bfs = true; }
+    ~BfsScope() { g->solving_bfs = false; }
+};
+
 void bfs(sssp_graph *g, int32_t source, int sched, sssp_stats *st)
 {
+    BfsScope scope(g);
     solve_t<WUnit>(g, &source, 1, sched, st);
 }
 
@@ -1681,8 +1693,14 @@ void certify_levels(sssp_graph *g, int32_t source, int64_t *counts)
 void collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *times,
                      int32_t *rows)
 {
-    if (g->wtype == SSSP_W_F32) collect_t<WF32>(g, source, cap, feats, times, rows);
-    else collect_t<WU32>(g, source, cap, feats, times, rows);
+    if (g->collect_kind == 1) {
+        BfsScope scope(g);
+        collect_t<WUnit>(g, source, cap, feats, times, rows);
+    } else if (g->wtype == SSSP_W_F32) {
+        collect_t<WF32>(g, source, cap, feats, times, rows);
+    } else {
+        collect_t<WU32>(g, source, cap, feats, times, rows);
+    }
 }
 
 void certify(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *counts)

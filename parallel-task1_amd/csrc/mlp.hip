@@ -385,4 +385,13 @@ int sssp_graph_set_mlp(sssp_graph *g, sssp_mlp *m)
     });
 }
 
+int sssp_graph_set_bfs_mlp(sssp_graph *g, sssp_mlp *m)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g, "null graph");
+        SSSP_REQUIRE(!m || m->device == g->device, "mlp and graph live on different devices");
+        g->bfs_mlp = m;
+    });
+}
+
 }  // extern "C"

@@ -144,6 +144,7 @@ struct sssp_graph {
     int hub_degree = 0;            // ... unless their out-degree is below this
     int64_t plain_below_edges = (int64_t)1 << 25;  // heuristic/learned: plain passes below this E
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
+    int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds
     int trace_events = 0;
 
     // last solve
@@ -153,7 +154,9 @@ struct sssp_graph {
     bool bfs_result = false;      // g->dist holds BFS levels of last_source
     std::vector<sssp::TraceRow> trace;
 
-    sssp_mlp *mlp = nullptr;
+    sssp_mlp *mlp = nullptr;      // scheduler network of SSSP solves
+    sssp_mlp *bfs_mlp = nullptr;  // scheduler network of BFS solves
+    bool solving_bfs = false;     // the rounds in flight are a BFS
 
     // batch state (allocated on demand)
     uint32_t *bdist = nullptr;

@@ -107,6 +107,7 @@ _SIGS = {
     "sssp_mlp_loss_grad": (I32, [P, P, P, P, I32, PF32, P]),
     "sssp_mlp_train": (I32, [P, P, P, P, I32, I32, I32, F32, U64, P]),
     "sssp_graph_set_mlp": (I32, [P, P]),
+    "sssp_graph_set_bfs_mlp": (I32, [P, P]),
     "sssp_collect_timings": (I32, [P, I32, I32, P, P, PI32]),
     "sssp_features": (I32, [F64, F64, F64, F64, F64, F64, F64, P]),
     "sssp_debug_fail_next_alloc": (I32, []),
@@ -390,6 +391,11 @@ class DeviceGraph:
     def set_mlp(self, mlp):
         _check(lib().sssp_graph_set_mlp(self._h, None if mlp is None else mlp.handle))
         self._mlp = mlp
+
+    def set_bfs_mlp(self, mlp):
+        """Scheduler network of learned BFS solves (None: the rule)."""
+        self._bfs_mlp = mlp
+        _check(lib().sssp_graph_set_bfs_mlp(self._h, mlp.handle if mlp else None))
 
     def solve_batch(self, sources, policy=0, download=True):
         sources = np.ascontiguousarray(sources, np.int32)

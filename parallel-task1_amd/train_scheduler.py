@@ -35,11 +35,19 @@ import sssp_amd as S  # noqa: E402
 MODELS = os.path.join(HERE, "models")
 
 
+KIND = "sssp"  # set from --kind before collecting
+
+
 def _rollout_with(g, net):
     """Rollouts continue with `net` (one more policy-improvement step) when
-    given, else with the hand-written rule."""
+    given, else with the hand-written rule; --kind bfs collects BFS rounds."""
+    if KIND == "bfs":
+        g.set_param("collect_kind", 1)
     if net is not None:
-        g.set_mlp(net)
+        if KIND == "bfs":
+            g.set_bfs_mlp(net)
+        else:
+            g.set_mlp(net)
         g.set_param("rollout_policy", 1)
 
 
@@ -132,7 +140,10 @@ def main():
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--out", default=os.path.join(MODELS, "scheduler_mlp.f32"))
+    ap.add_argument("--kind", choices=("sssp", "bfs"), default="sssp",
+                    help="rounds of SSSP solves or of BFS (unit weights)")
+    ap.add_argument("--out", help="default: models/scheduler_mlp.f32 (sssp), "
+                                  "models/scheduler_bfs_mlp.f32 (bfs)")
     ap.add_argument("--data", help="train on the X/T arrays of an earlier run's _data.npz")
     ap.add_argument("--rollout-weights",
                     help="weights of a network the rollouts continue with (policy iteration); "
@@ -140,6 +151,11 @@ def main():
     ap.add_argument("--min-spread", type=float, default=1.02)
     ap.add_argument("--max-copies", type=int, default=8)
     args = ap.parse_args()
+    global KIND
+    KIND = args.kind
+    if not args.out:
+        args.out = os.path.join(MODELS, "scheduler_mlp.f32" if KIND == "sssp"
+                                else "scheduler_bfs_mlp.f32")
 
     if args.data:
         with np.load(args.data) as z:
@@ -175,6 +191,7 @@ def main():
     pc = out[:, S.MLP_MODES:].argmax(axis=1)
     # the heuristic's choice on the same rows, for comparison (fixed chunks)
     report = {
+        "kind": args.kind,
         "rows": int(n), "train_rows": int(len(tr)), "val_rows": int(nval),
         "train_rows_kept": int((copies > 0).sum()), "train_rows_weighted": int(len(trw)),
         "min_spread": args.min_spread, "max_copies": args.max_copies,

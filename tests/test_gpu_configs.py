@@ -29,6 +29,7 @@ import sssp_amd as S
 from conftest import ROOT, bits
 
 WEIGHTS = os.path.join(ROOT, "parallel-task1_amd", "models", "scheduler_mlp.f32")
+BFS_WEIGHTS = os.path.join(ROOT, "parallel-task1_amd", "models", "scheduler_bfs_mlp.f32")
 
 
 def _with_network(g):
@@ -97,6 +98,10 @@ def test_c2_rmat22_ef32_sixteen_sources(gpu, oracle):
         dc, passes, sec = _cpu_bf(oracle, src, dst, w, V, int(sources[0]))
         assert np.array_equal(bits(dc), bits(d))
         # BFS on the same graph and engine, every schedule against the oracle
+        if os.path.exists(BFS_WEIGHTS):
+            mb = S.MLP(0)
+            mb.set_params(np.fromfile(BFS_WEIGHTS, np.float32))
+            g.set_bfs_mlp(mb)
         vref, lref, nref = oracle.bfs(src, dst, V, int(sources[0]))
         tb = {}
         for sched in ("learned", "heuristic", "plain"):

@@ -126,3 +126,30 @@ def test_trained_network_against_heuristic(gpu):
         assert gain > 0.8
     finally:
         g.close()
+
+
+@pytest.mark.gpu
+def test_bfs_network_slot(gpu, oracle):
+    """BFS has its own network slot: without one a learned BFS runs the rule
+    (the SSSP network is not used); with any network the levels stay the
+    hop distances."""
+    src, dst, w = S.rmat(13, 16, seed=21, threads=8)
+    V = 1 << 13
+    _, lev_ref, _ = oracle.bfs(src, dst, V, 0)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
+        g.set_mlp(S.MLP(0, seed=1))          # an SSSP network: ignored by BFS
+        _, lev, st = g.bfs(0, "learned")
+        assert np.array_equal(lev, lev_ref) and st["mlp_rounds"] == 0
+        for seed in range(2):
+            g.set_bfs_mlp(S.MLP(0, seed=seed + 5))
+            _, lev, st = g.bfs(0, "learned")
+            assert np.array_equal(lev, lev_ref) and st["mlp_rounds"] > 0
+        # BFS timing collection (rollouts) leaves a certified BFS result
+        g.set_param("collect_kind", 1)
+        feats, times = g.collect_timings(0)
+        assert len(feats) > 0 and np.all(times[:, :8] > 0)
+        ok, counts, _ = g.certify(0)
+        assert ok, counts
+        g.set_mlp(None)
+        g.set_bfs_mlp(None)
