@@ -424,10 +424,27 @@ class BFS {
     void set_omp_threads(int n) { omp_threads_ = n; }
     void set_schedule(int schedule) { schedule_ = schedule; }
 
+    ~BFS()
+    {
+        if (mlp_) sssp_mlp_destroy(mlp_);
+    }
+    BFS(const BFS &) = delete;
+    BFS &operator=(const BFS &) = delete;
+
+    // Loads BFS scheduler weights (SSSP_MLP_NPARAM floats, e.g.
+    // models/scheduler_bfs_mlp.f32) for SSSP_SCHED_LEARNED.
+    void set_scheduler_weights(const float *params)
+    {
+        if (!mlp_) sssp_host::check(sssp_mlp_create(device_, 0, &mlp_));
+        sssp_host::check(sssp_mlp_set_params(mlp_, params));
+        if (cache_.current()) sssp_host::check(sssp_graph_set_bfs_mlp(cache_.current(), mlp_));
+    }
+
     double gpu_bfs(Graph<_TVertexValue, _TEdgeWeight> &graph, bool *result, int source_vertex)
     {
         static_assert(sizeof(bool) == 1, "bool must be one byte, as the visited array");
         sssp_graph *g = cache_.get(graph, device_, NULL);
+        if (mlp_) sssp_host::check(sssp_graph_set_bfs_mlp(g, mlp_));
         sssp_stats st;
         sssp_host::check(sssp_bfs(g, source_vertex, schedule_, reinterpret_cast<uint8_t *>(result),
                                   NULL, &st));
@@ -438,5 +455,6 @@ class BFS {
     int omp_threads_;
     int device_;
     int schedule_;
+    sssp_mlp *mlp_ = NULL;
     sssp_host::DeviceGraphCache<_TVertexValue, _TEdgeWeight> cache_;
 };
