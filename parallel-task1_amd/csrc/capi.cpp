@@ -152,8 +152,12 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     g->parent = dmalloc<int32_t>(V);
     g->ctrl = dmalloc<Ctrl>(1);
     SSSP_HIP(hipMemsetAsync(g->ctrl, 0, sizeof(Ctrl), s));
-    SSSP_HIP(hipHostMalloc((void **)&g->ctrl_host, sizeof(Ctrl), hipHostMallocDefault));
+    // coherent and mapped: the decision kernel writes the round's counters
+    // straight into it, so a round costs one stream synchronize, no copies
+    SSSP_HIP(hipHostMalloc((void **)&g->ctrl_host, sizeof(Ctrl),
+                           hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(g->ctrl_host, 0, sizeof(Ctrl));
+    SSSP_HIP(hipHostGetDevicePointer((void **)&g->ctrl_host_dev, g->ctrl_host, 0));
     // fixed-stride vertex sample for the unsettled-fraction estimate
     g->n_sample = g->V < 16384 ? g->V : 16384;
     std::vector<int32_t> sv((size_t)g->n_sample);

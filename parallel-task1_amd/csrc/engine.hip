@@ -774,6 +774,7 @@ struct DecideArgs {
     int push_chunk, all_chunk, pull_chunk;
     const float *mlp;  // nullable
     int pile;          // pile buffer the round reads
+    Ctrl *mirror;      // device address of the host mirror (coherent host memory)
 };
 
 // One wave: features, the hand-written rule and (learned schedule) the MLP.
@@ -858,6 +859,20 @@ __global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
         c->dec_heur_mode = hmode;
         for (int i = 0; i < kIn; i++) c->feat[i] = s_f[i];
         for (int i = 0; i < kOut; i++) c->logits[i] = by_mlp ? s_o[i] : 0.f;
+        // what the host reads to launch the round, written straight into
+        // host memory (visible once the kernel has completed)
+        Ctrl *m = d.mirror;
+        m->slot[d.r & 1] = *cur;
+        m->pile_n[d.pile] = c->pile_n[d.pile];
+        m->pile_min[d.pile] = c->pile_min[d.pile];
+        m->pile_edges[d.pile] = c->pile_edges[d.pile];
+        m->lb_n = cur->qlen;
+        m->dec_mode = mode;
+        m->dec_chunk = chunk;
+        m->dec_mlp = by_mlp;
+        m->dec_heur_mode = hmode;
+        for (int i = 0; i < kIn; i++) m->feat[i] = s_f[i];
+        for (int i = 0; i < kOut; i++) m->logits[i] = by_mlp ? s_o[i] : 0.f;
     }
 }
 
@@ -1295,9 +1310,10 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     sssp_mlp *net = g->solving_bfs ? g->bfs_mlp : g->mlp;
     da.mlp = (learned && net) ? mlp_device_params(net) : nullptr;
     da.pile = g->pile_cur;
+    da.mirror = g->ctrl_host_dev;
     k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
     SSSP_LAUNCH_CHECK("k_decide");
-    read_slot(g, r);
+    SSSP_HIP(hipStreamSynchronize(s));
     const RoundSlot cur = g->ctrl_host->slot[r & 1];
     const int32_t piled = g->ctrl_host->pile_n[g->pile_cur];
     const int64_t piled_edges = g->ctrl_host->pile_edges[g->pile_cur];

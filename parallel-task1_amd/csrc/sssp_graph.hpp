@@ -129,7 +129,8 @@ struct sssp_graph {
     int pile_cur = 0;             // pile buffer the coming round reads
     bool pile_dirty = false;      // a solve stopped with vertices still piled
     sssp::Ctrl *ctrl = nullptr;
-    sssp::Ctrl *ctrl_host = nullptr;  // pinned mirror
+    sssp::Ctrl *ctrl_host = nullptr;  // pinned mirror (coherent, mapped)
+    sssp::Ctrl *ctrl_host_dev = nullptr;  // the mirror's device address
     int32_t tick = 1;
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_r0 = nullptr, ev_r1 = nullptr;
 
