@@ -43,7 +43,7 @@ void graph_free(sssp_graph *g)
                     g->queue[0], g->queue[1], g->lb_v, g->lb_start, g->lb_scan, g->lb_tile,
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
-                    g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst};
+                    g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {
@@ -142,6 +142,7 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     g->pile[0] = dmalloc<int32_t>(V);
     g->pile[1] = dmalloc<int32_t>(V);
     g->farst = dmalloc<int32_t>(V);
+    g->snap = dmalloc<uint32_t>(V);
     reset_stamps(g);
     g->queue[0] = dmalloc<int32_t>(V + 1);
     g->queue[1] = dmalloc<int32_t>(V + 1);
@@ -519,6 +520,12 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "rollout_policy") {
             SSSP_REQUIRE(value == 0 || value == 1, "rollout_policy must be 0 (rule) or 1 (network)");
             g->rollout_policy = (int)value;
+        } else if (n == "dense_unreached") {
+            SSSP_REQUIRE(value >= 0 && value <= 2, "dense_unreached must be in [0, 2]");
+            g->dense_unreached = value;
+        } else if (n == "dense_all_frac") {
+            SSSP_REQUIRE(value >= 0 && value <= 1e9, "dense_all_frac must be >= 0");
+            g->dense_all_frac = value;
         } else if (n == "plain_below_edges") {
             SSSP_REQUIRE(value >= 0 && value <= 9.2e18, "plain_below_edges must be >= 0");
             g->plain_below_edges = (int64_t)value;
@@ -546,6 +553,8 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "delta") *value = g->delta;
         else if (n == "hub_degree") *value = g->hub_degree;
         else if (n == "plain_below_edges") *value = (double)g->plain_below_edges;
+        else if (n == "dense_all_frac") *value = g->dense_all_frac;
+        else if (n == "dense_unreached") *value = g->dense_unreached;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;
         else if (n == "trace_events") *value = g->trace_events;

@@ -24,6 +24,7 @@
 //   converge:  stop after a pass that changes nothing          (:126-141)
 // The plain schedule (SSSP_SCHED_PLAIN) is exactly that loop over the
 // src-sorted edge list, with the convergence flag tested on the device.
+#include <algorithm>
 #include <cstring>
 
 #include "mlp_dev.hpp"
@@ -679,7 +680,7 @@ __device__ void zero_slot(RoundSlot *s, uint32_t minw)
     s->qedges = 0;
     s->relaxed = 0;
     s->lowered = 0;
-    s->reserved0 = 0;
+    s->sample_unreached = 0;
     s->sample_unsettled = 0;
     s->sample_n = 0;
     s->sample_unsettled_in = 0;
@@ -737,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_init_sources(Ctrl *c, uint32_t *dist
 __global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r, int pile,
                                                         const int32_t *sample_v, int32_t n_sample,
                                                         const uint32_t *dist,
-                                                        const int64_t *col_ptr)
+                                                        const int64_t *col_ptr, uint32_t inf)
 {
     RoundSlot *cur = &c->slot[r & 1];
     RoundSlot *nxt = &c->slot[(r + 1) & 1];
@@ -746,20 +747,24 @@ __global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r, int 
         cur->sample_n = n_sample;
     }
     const uint32_t minF = pile >= 0 ? min(cur->minw, c->pile_min[pile]) : cur->minw;
-    int cnt = 0;
+    int cnt = 0, unr = 0;
     int64_t ind = 0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_sample;
          i += (int64_t)gridDim.x * kBlock) {
         const int32_t v = sample_v[i];
-        if (dist[v] > minF) {
+        const uint32_t d = dist[v];
+        if (d > minF) {
             cnt++;
             if (col_ptr) ind += col_ptr[v + 1] - col_ptr[v];
         }
+        unr += d == inf ? 1 : 0;
     }
     cnt = wave_sum32(cnt);
+    unr = wave_sum32(unr);
     ind = wave_sum64(ind);
     if (__lane_id() == 0) {
         if (cnt) atomicAdd(&cur->sample_unsettled, cnt);
+        if (unr) atomicAdd(&cur->sample_unreached, unr);
         if (ind) atomicAdd((unsigned long long *)&cur->sample_unsettled_in, (unsigned long long)ind);
     }
 }
@@ -966,6 +971,48 @@ __global__ __launch_bounds__(kBlock) void k_el_pass(const int32_t *__restrict__ 
     if (__ballot(low) != 0 && __lane_id() == 0) changed[pass & 63] = 1;
 }
 
+// Frontier of a dense all-edge round (the plain pass over every reached
+// source, without appends): the vertices the round lowered -- distance below
+// the snapshot taken before it -- that have out-edges join the next queue with
+// the round's stamp, the queue and stamps the edge loop's appends would have
+// produced. Also fills the round's counters (lowered vertices, their minimum).
+__global__ __launch_bounds__(kBlock) void k_lowered_compact(const uint32_t *dist,
+                                                            const uint32_t *snap, int32_t *stamp,
+                                                            int32_t tick_cur,
+                                                            const int64_t *row_ptr, int64_t V,
+                                                            int64_t relaxed, int32_t *qnext,
+                                                            RoundSlot *slot)
+{
+    __shared__ int32_t s_stage[kBlock / 64][kStageCap];
+    int32_t *wbuf = s_stage[threadIdx.x >> 6];
+    int wcnt = 0;
+    RoundAcc acc;
+    if (blockIdx.x == 0 && threadIdx.x == 0) acc.relaxed = relaxed;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    // block-uniform trip count: every lane of a wave reaches stage_push
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < V; base += stride) {
+        const int64_t v = base + threadIdx.x;
+        bool app = false;
+        if (v < V) {
+            const uint32_t d = dist[v];
+            if (d < snap[v]) {
+                acc.lowered++;
+                acc.minw = min(acc.minw, d);
+                const int32_t sv = stamp[v];
+                if (sv != kSinkStamp && sv != tick_cur) {
+                    stamp[v] = tick_cur;  // only this lane handles v
+                    app = true;
+                    acc.qedges += row_ptr[v + 1] - row_ptr[v];
+                }
+            }
+        }
+        stage_push(app, (int32_t)v, wbuf, wcnt);
+        if (wcnt > kStageCap - 64) stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+    }
+    stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+    flush_acc(acc, slot);
+}
+
 // ---------------------------------------------------------------------------
 // Certificate helpers.
 // ---------------------------------------------------------------------------
@@ -1153,6 +1200,28 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
         SSSP_LAUNCH_CHECK("k_pile_flush");
     }
     if (verify || mode == SSSP_MODE_ALL) {
+        const double fe = (double)cur.qedges + (pn > 0 ? (double)h.pile_edges[p] : 0.0);
+        const double unreached =
+            cur.sample_n > 0 ? (double)cur.sample_unreached / cur.sample_n : 0.0;
+        if (verify || fe >= g->dense_all_frac * (double)g->E || unreached >= g->dense_unreached) {
+            // dense all-edge round: with most edges active -- a large
+            // frontier, or most of the graph still unreached so that the sweep
+            // lowers most vertices it meets -- the plain pass (every reached
+            // source, no per-edge appends, the larger atomic filter) and a
+            // compaction of the lowered vertices against a snapshot cost less
+            // than the edge loop's appends
+            SSSP_HIP(hipMemcpyAsync(g->snap, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToDevice, s));
+            const int64_t nb = std::max<int64_t>(8, (int64_t)g->cu_count * 8);
+            const int64_t per_block = ((g->E >> 2) + nb - 1) / nb;
+            k_el_pass<WT, uint32_t, 4, true><<<(unsigned)nb, kBlock, 0, s>>>(
+                g->x_src, g->x_dst, g->x_w, g->E, g->dist, g->ctrl->changed, 0, per_block,
+                g->xgoff);
+            SSSP_LAUNCH_CHECK("k_el_pass (dense round)");
+            k_lowered_compact<<<grid_for(g->V, (int64_t)g->cu_count * 8), kBlock, 0, s>>>(
+                g->dist, g->snap, g->stamp, a.tick_cur, g->row_ptr, g->V, g->E, a.qnext, a.slot);
+            SSSP_LAUNCH_CHECK("k_lowered_compact");
+            return false;
+        }
         // all-edge round streamed over the src-sorted edge list; the chunk
         // bucket sets the grid (2, 4, 8 or 16 workgroups per CU)
         const int64_t nblocks = (int64_t)g->cu_count * (2 << kidx);
@@ -1294,7 +1363,8 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     const bool learned = (sched == SSSP_SCHED_LEARNED);
     const int ns = g->n_sample;
     k_round_begin<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v, ns,
-                                                      g->dist, g->has_csc ? g->col_ptr : nullptr);
+                                                      g->dist, g->has_csc ? g->col_ptr : nullptr,
+                                                      WT::INF);
     SSSP_LAUNCH_CHECK("k_round_begin");
     DecideArgs da{};
     da.r = r;
@@ -1485,7 +1555,8 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
     SSSP_LAUNCH_CHECK("k_cert_roots");
     for (int32_t r = 0;; r++) {
         if (r > g->V + 2) throw Error{SSSP_ERR_INTERNAL, "certificate BFS did not terminate"};
-        k_round_begin<<<1, kBlock, 0, s>>>(c, r, -1, g->sample_v, 0, g->dist, nullptr);
+        k_round_begin<<<1, kBlock, 0, s>>>(c, r, -1, g->sample_v, 0, g->dist, nullptr,
+                                           0xFFFFFFFFu);
         SSSP_LAUNCH_CHECK("k_round_begin (bfs)");
         read_slot(g, r);
         const RoundSlot cur = g->ctrl_host->slot[r & 1];

@@ -27,8 +27,8 @@ struct alignas(64) RoundSlot {
     int64_t qedges;        // sum of out-degrees of appended vertices
     int64_t relaxed;       // edge relaxations evaluated
     int32_t lowered;       // relaxations whose candidate beat the distance read
-    int32_t reserved0;
     // filled by the round-begin kernel for the consuming round:
+    int32_t sample_unreached;   // sampled vertices with d = INF
     int32_t sample_unsettled;   // sampled vertices with d > minF
     int32_t sample_n;           // sample size
     int64_t sample_unsettled_in;  // sum of in-degrees of the unsettled sample
@@ -126,6 +126,7 @@ struct sssp_graph {
     int32_t *parent = nullptr;    // certificate tight tree
     int32_t *pile[2] = {nullptr, nullptr};  // far pile (push rounds), double-buffered
     int32_t *farst = nullptr;     // per-vertex pile state (kPileIn / tick moved out)
+    uint32_t *snap = nullptr;     // distances before a dense all-edge round
     int pile_cur = 0;             // pile buffer the coming round reads
     bool pile_dirty = false;      // a solve stopped with vertices still piled
     sssp::Ctrl *ctrl = nullptr;
@@ -144,6 +145,8 @@ struct sssp_graph {
     double delta = 0.0;            // push rounds defer vertices with d >= min + delta (0 = off)
     int hub_degree = 0;            // ... unless their out-degree is below this
     int64_t plain_below_edges = (int64_t)1 << 25;  // heuristic/learned: plain passes below this E
+    double dense_all_frac = 0.7;  // all-edge rounds with frontier edges >= this * E run dense,
+    double dense_unreached = 0.5;  // ... and so do those with this sampled unreached fraction
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds
     int trace_events = 0;

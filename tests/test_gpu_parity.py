@@ -411,3 +411,37 @@ def test_small_graphs_hand_the_rule_and_network_to_plain_passes(gpu, oracle):
         g.set_param("plain_below_edges", 0)
         d, st = g.solve(0, "heuristic")
         assert np.array_equal(bits(d), bits(ref)) and st["n_push"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("frac,unreached", [(0.0, 2.0), (1e9, 0.0), (1e9, 2.0), (0.7, 0.5)])
+def test_dense_all_edge_rounds_keep_the_fixpoint(gpu, oracle, frac, unreached):
+    """All-edge rounds run dense (plain pass, then the lowered vertices are
+    compacted against a snapshot into the next queue) when the frontier is
+    large or most of the graph is unreached; forced on for every all-edge
+    round, on only by the unreached rule, and off (the fixpoint check stays
+    dense), the distances, levels and frontiers stay exact."""
+    src, dst, w = S.rmat(14, 16, seed=41, threads=16)
+    V = 1 << 14
+    su, tu, wu = S.er(1 << 13, 8, seed=13, weight_mode=1)
+    with S.DeviceGraph(src, dst, w, V) as g, S.DeviceGraph(su, tu, wu, 1 << 13) as gu:
+        for h in (g, gu):
+            h.set_param("plain_below_edges", 0)  # exercise the round engine
+            h.set_param("dense_all_frac", frac)
+            h.set_param("dense_unreached", unreached)
+            assert h.get_param("dense_all_frac") == frac
+        for s in (0, int(src[77])):
+            ref = oracle.dijkstra(src, dst, w, V, s)
+            for sched in ("all", "heuristic", "learned", "push"):
+                d, st = g.solve(s, sched)
+                assert np.array_equal(bits(d), bits(ref)), (frac, unreached, s, sched)
+                assert st["reached"] == int((d != S.F32_INF).sum())
+            ok, counts, _ = g.certify(s)
+            assert ok, counts
+            _, lev, _ = g.bfs(s, "all")
+            assert np.array_equal(lev, oracle.bfs(src, dst, V, s)[1])
+        for sched in ("all", "heuristic"):
+            d, _ = gu.solve(0, sched)
+            assert np.array_equal(d, oracle.dijkstra(su, tu, wu, 1 << 13, 0)), sched
+        with pytest.raises(S.SSSPError):
+            g.set_param("dense_unreached", -1)
