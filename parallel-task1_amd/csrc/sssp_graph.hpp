@@ -127,6 +127,7 @@ struct sssp_graph {
     int32_t *pile[2] = {nullptr, nullptr};  // far pile (push rounds), double-buffered
     int32_t *farst = nullptr;     // per-vertex pile state (kPileIn / tick moved out)
     uint32_t *snap = nullptr;     // distances before a dense all-edge round
+    int32_t snap_round = -1;      // snap equals dist as dense round snap_round left it (-1: stale)
     int pile_cur = 0;             // pile buffer the coming round reads
     bool pile_dirty = false;      // a solve stopped with vertices still piled
     sssp::Ctrl *ctrl = nullptr;
