@@ -43,7 +43,8 @@ void graph_free(sssp_graph *g)
                     g->queue[0], g->queue[1], g->lb_v, g->lb_start, g->lb_scan, g->lb_tile,
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
-                    g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap};
+                    g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap,
+                    g->b_src, g->b_dst, g->b_w, g->b_tile};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {
@@ -520,6 +521,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "rollout_policy") {
             SSSP_REQUIRE(value == 0 || value == 1, "rollout_policy must be 0 (rule) or 1 (network)");
             g->rollout_policy = (int)value;
+        } else if (n == "dense_kernel") {
+            SSSP_REQUIRE(value == 0 || value == 1, "dense_kernel must be 0 or 1");
+            g->dense_kernel = (int)value;
         } else if (n == "dense_unreached") {
             SSSP_REQUIRE(value >= 0 && value <= 2, "dense_unreached must be in [0, 2]");
             g->dense_unreached = value;
@@ -555,6 +559,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "plain_below_edges") *value = (double)g->plain_below_edges;
         else if (n == "dense_all_frac") *value = g->dense_all_frac;
         else if (n == "dense_unreached") *value = g->dense_unreached;
+        else if (n == "dense_kernel") *value = g->dense_kernel;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;
         else if (n == "trace_events") *value = g->trace_events;

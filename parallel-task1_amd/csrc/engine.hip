@@ -74,11 +74,12 @@ struct RoundAcc {
     uint32_t minw = 0xFFFFFFFFu;
 };
 
-__device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
+template <int NW>
+__device__ __forceinline__ void flush_acc_n(RoundAcc &acc, RoundSlot *s)
 {
-    __shared__ int64_t r_rel[kBlock / 64], r_qe[kBlock / 64];
-    __shared__ int r_low[kBlock / 64];
-    __shared__ uint32_t r_min[kBlock / 64];
+    __shared__ int64_t r_rel[NW], r_qe[NW];
+    __shared__ int r_low[NW];
+    __shared__ uint32_t r_min[NW];
     const int lane = __lane_id(), w = threadIdx.x >> 6;
     const int64_t rel = wave_sum64(acc.relaxed), qe = wave_sum64(acc.qedges);
     const int low = wave_sum32(acc.lowered);
@@ -94,7 +95,7 @@ __device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
         int64_t a = 0, b = 0;
         int c = 0;
         uint32_t m = 0xFFFFFFFFu;
-        for (int i = 0; i < kBlock / 64; i++) {
+        for (int i = 0; i < NW; i++) {
             a += r_rel[i];
             b += r_qe[i];
             c += r_low[i];
@@ -105,6 +106,11 @@ __device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
         if (c) atomicAdd(&s->lowered, c);
         if (m != 0xFFFFFFFFu) atomicMin(&s->minw, m);
     }
+}
+
+__device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
+{
+    flush_acc_n<kBlock / 64>(acc, s);
 }
 
 // A vertex lowered in this round joins the next frontier when it has
@@ -1018,6 +1024,104 @@ __global__ __launch_bounds__(kBlock) void k_lowered_compact(const uint32_t *dist
     flush_acc(acc, slot);
 }
 
+// Dense all-edge round over the destination-blocked edge list. A workgroup
+// takes one tile: it loads its block's 2^kBlkBits distances into LDS, relaxes
+// the tile's edges into them (d[src] gathered from global memory, the minimum
+// taken with LDS atomics), then writes back the vertices it lowered and
+// appends them to the next frontier with the round's stamp. A block with one
+// tile belongs to one workgroup: plain stores, plain stamps, no global
+// atomics in the whole round. Tiles of a split block (a high in-degree
+// block) write back with a returning atomicMin and stamp with atomicExch, so
+// each lowered vertex is appended once. Fills the round's counters.
+constexpr int kBlkThreads = 1024;
+constexpr int kBlkStageCap = 128;  // per-wave LDS staging slots
+
+template <class WT>
+__global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
+    const BlkTile *__restrict__ tiles, const int32_t *__restrict__ bsrc,
+    const int32_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
+    int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, int32_t *qnext, RoundSlot *slot)
+{
+    typedef int32_t v4i __attribute__((ext_vector_type(4)));
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    constexpr int NB = 1 << kBlkBits;
+    constexpr int PER = NB / kBlkThreads;  // block vertices per thread
+    constexpr uint32_t INF = WT::INF;
+    __shared__ uint32_t s_d[NB];
+    __shared__ int32_t s_stage[kBlkThreads / 64][kBlkStageCap];
+    int32_t *wbuf = s_stage[threadIdx.x >> 6];
+    int wcnt = 0;
+    RoundAcc acc;
+    const BlkTile t = tiles[blockIdx.x];
+    const int64_t base = (int64_t)t.blk << kBlkBits;
+    const int nb = (int)min((int64_t)NB, V - base);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = k * kBlkThreads + threadIdx.x;
+        s_d[i] = i < nb ? dist[base + i] : INF;
+    }
+    __syncthreads();
+    const v4i *s4 = reinterpret_cast<const v4i *>(bsrc);
+    const v4i *t4 = reinterpret_cast<const v4i *>(bdst);
+    const v4u *w4 = reinterpret_cast<const v4u *>(bw);
+    auto relax = [&](uint32_t du, int32_t v, uint32_t ww) {
+        if (du == INF) return;
+        acc.relaxed++;
+        const uint32_t cand = WT::add(du, ww);
+        const int loc = (int)(v - base);
+        if (cand < s_d[loc]) atomicMin(&s_d[loc], cand);
+    };
+    for (int64_t g = (t.e0 >> 2) + threadIdx.x; g < (t.e1 >> 2); g += kBlkThreads) {
+        const v4i sv = __builtin_nontemporal_load(&s4[g]);
+        const v4i tv = __builtin_nontemporal_load(&t4[g]);
+        const v4u wv = __builtin_nontemporal_load(&w4[g]);
+        // src-sorted inside the block: equal sources are loaded once
+        const uint32_t u0 = dist[sv.x];
+        const uint32_t u1 = sv.y == sv.x ? u0 : dist[sv.y];
+        const uint32_t u2 = sv.z == sv.y ? u1 : dist[sv.z];
+        const uint32_t u3 = sv.w == sv.z ? u2 : dist[sv.w];
+        relax(u0, tv.x, wv.x);
+        relax(u1, tv.y, wv.y);
+        relax(u2, tv.z, wv.z);
+        relax(u3, tv.w, wv.w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {  // block-uniform: every lane reaches stage_push
+        const int i = k * kBlkThreads + threadIdx.x;
+        const int32_t v = (int32_t)(base + i);
+        bool app = false;
+        if (i < nb) {
+            // the block's distance now: the value loaded above unless another
+            // tile of a split block lowered it since
+            const uint32_t nv = s_d[i];
+            if (nv < dist[v]) {
+                bool low = true;
+                if (t.split) low = atomicMin(&dist[v], nv) > nv;
+                else dist[v] = nv;  // only this workgroup writes the block
+                if (low) {
+                    acc.lowered++;
+                    acc.minw = min(acc.minw, nv);
+                    const int64_t deg = row_ptr[v + 1] - row_ptr[v];
+                    if (deg > 0) {
+                        if (t.split) {
+                            app = atomicExch(&stamp[v], tick_cur) != tick_cur;
+                        } else {
+                            stamp[v] = tick_cur;
+                            app = true;
+                        }
+                        if (app) acc.qedges += deg;
+                    }
+                }
+            }
+        }
+        stage_push(app, v, wbuf, wcnt);
+        if (wcnt > kBlkStageCap - 64) stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+    }
+    stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+    flush_acc_n<kBlkThreads / 64>(acc, slot);
+}
+
 // ---------------------------------------------------------------------------
 // Certificate helpers.
 // ---------------------------------------------------------------------------
@@ -1213,10 +1317,19 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
         if (verify || fe >= g->dense_all_frac * (double)g->E || unreached >= g->dense_unreached) {
             // dense all-edge round: with most edges active -- a large
             // frontier, or most of the graph still unreached so that the sweep
-            // lowers most vertices it meets -- the plain pass (every reached
-            // source, no per-edge appends, the larger atomic filter) and a
-            // compaction of the lowered vertices against a snapshot cost less
-            // than the edge loop's appends
+            // lowers most vertices it meets -- relaxing every reached source
+            // without per-edge appends costs less than the edge loop's
+            // appends: over the destination-blocked list (LDS minima, no
+            // global atomics, the frontier written by the same kernel), or
+            // the plain pass followed by a compaction of the lowered vertices
+            // against a snapshot
+            if (g->dense_kernel == 0 && g->n_btile > 0) {
+                k_blk_pass<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, s>>>(
+                    g->b_tile, g->b_src, g->b_dst, g->b_w, g->dist, g->V, g->stamp, a.tick_cur,
+                    g->row_ptr, a.qnext, a.slot);
+                SSSP_LAUNCH_CHECK("k_blk_pass");
+                return false;
+            }
             // the snapshot still holds the distances a dense round r - 1 left
             // (its compaction updated it) unless anything else ran since
             if (!snap_ok)

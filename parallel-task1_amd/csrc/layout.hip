@@ -253,6 +253,49 @@ __global__ __launch_bounds__(kBlock) void k_group_gather(const uint32_t *perm, c
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_blk_keys(const int32_t *dst, int64_t E, uint32_t *keys)
+{
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * kBlock)
+        keys[e] = (uint32_t)dst[e] >> kBlkBits;
+}
+
+// Sorted position j of block b = skeys[j] goes to boff[b] + (j - bptr[b]);
+// perm[j] is its position in the src-sorted list, whose order is kept.
+__global__ __launch_bounds__(kBlock) void k_blk_gather(const uint32_t *skeys, const uint32_t *perm,
+                                                       int64_t E, const int64_t *bptr,
+                                                       const int64_t *boff, const int32_t *src,
+                                                       const int32_t *dst, const uint32_t *w,
+                                                       int32_t *bs, int32_t *bd, uint32_t *bw)
+{
+    for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < E;
+         j += (int64_t)gridDim.x * kBlock) {
+        const uint32_t b = skeys[j];
+        const int64_t o = boff[b] + (j - bptr[b]);
+        const uint32_t p = perm[j];
+        bs[o] = src[p];
+        bd[o] = dst[p];
+        bw[o] = w[p];
+    }
+}
+
+// Up to 3 padding slots per block: weight-0 self loops on the block's first
+// vertex (a candidate equal to the distance it would lower: no change).
+__global__ __launch_bounds__(kBlock) void k_blk_pad(const int64_t *bptr, const int64_t *boff,
+                                                    int32_t nblk, int32_t *bs, int32_t *bd,
+                                                    uint32_t *bw)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)nblk * 4) return;
+    const int32_t b = (int32_t)(i >> 2);
+    const int64_t o = boff[b] + (bptr[b + 1] - bptr[b]) + (i & 3);
+    if (o < boff[b + 1]) {
+        bs[o] = b << kBlkBits;
+        bd[o] = b << kBlkBits;
+        bw[o] = 0u;
+    }
+}
+
 // row_ptr[v] = first position whose sorted key is >= v; row_ptr[V] = E.
 __global__ __launch_bounds__(kBlock) void k_rowptr_from_sorted(const uint32_t *keys, int64_t E,
                                                                int32_t V, int64_t *row_ptr)
@@ -531,6 +574,64 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
                                                                 g->x_dst, g->x_w);
             SSSP_LAUNCH_CHECK("k_group_gather");
         }
+        SSSP_HIP(hipStreamSynchronize(s));
+    }
+
+    // ---- destination-blocked edge list ----
+    if (E > 0) {
+        const int32_t nblk = (int32_t)(((int64_t)V + (1 << kBlkBits) - 1) >> kBlkBits);
+        uint32_t *keys = dmalloc<uint32_t>((size_t)E);
+        int64_t *bptr = dmalloc<int64_t>((size_t)nblk + 1);
+        int64_t *boff = dmalloc<int64_t>((size_t)nblk + 1);
+        struct F3 {
+            void *p[3];
+            ~F3()
+            {
+                for (void *x : p) dfree(x);
+            }
+        } f3{{keys, bptr, boff}};
+        k_blk_keys<<<grid_for(E), kBlock, 0, s>>>(g->el_dst, E, keys);
+        SSSP_LAUNCH_CHECK("k_blk_keys");
+        uint32_t *bkeys = nullptr, *bperm = nullptr;
+        radix_sort_pairs(reinterpret_cast<const int32_t *>(keys), E, nblk - 1, kA, kB, vA, vB,
+                         counts, g->blk_a, g->blk_b, g->n_blk, s, &bkeys, &bperm);
+        k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(bkeys, E, nblk, bptr);
+        SSSP_LAUNCH_CHECK("k_rowptr_from_sorted blk");
+        std::vector<int64_t> bp((size_t)nblk + 1), bo((size_t)nblk + 1);
+        SSSP_HIP(hipMemcpyAsync(bp.data(), bptr, 8 * bp.size(), hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        bo[0] = 0;
+        std::vector<BlkTile> tiles;
+        for (int32_t b = 0; b < nblk; b++) {
+            const int64_t c = (bp[(size_t)b + 1] - bp[(size_t)b] + 3) / 4 * 4;
+            bo[(size_t)b + 1] = bo[(size_t)b] + c;
+            const int64_t nt = (c + kBlkTileEdges - 1) / kBlkTileEdges;
+            for (int64_t k = 0; k < nt; k++) {
+                const int64_t e0 = bo[(size_t)b] + k * kBlkTileEdges;
+                tiles.push_back(BlkTile{b, nt > 1 ? 1 : 0, e0,
+                                        std::min(e0 + kBlkTileEdges, bo[(size_t)b + 1])});
+            }
+        }
+        // largest tiles first: the dispatch order, so the grid drains evenly
+        std::stable_sort(tiles.begin(), tiles.end(), [](const BlkTile &a, const BlkTile &b) {
+            return a.e1 - a.e0 > b.e1 - b.e0;
+        });
+        const int64_t Ep = bo[(size_t)nblk];
+        SSSP_HIP(hipMemcpyAsync(boff, bo.data(), 8 * bo.size(), hipMemcpyHostToDevice, s));
+        g->b_src = dmalloc<int32_t>((size_t)Ep);
+        g->b_dst = dmalloc<int32_t>((size_t)Ep);
+        g->b_w = dmalloc<uint32_t>((size_t)Ep);
+        g->n_btile = (int32_t)tiles.size();
+        g->b_tile = dmalloc<BlkTile>(tiles.size());
+        SSSP_HIP(hipMemcpyAsync(g->b_tile, tiles.data(), sizeof(BlkTile) * tiles.size(),
+                                hipMemcpyHostToDevice, s));
+        k_blk_gather<<<grid_for(E), kBlock, 0, s>>>(bkeys, bperm, E, bptr, boff, g->el_src,
+                                                    g->el_dst, g->el_w, g->b_src, g->b_dst,
+                                                    g->b_w);
+        SSSP_LAUNCH_CHECK("k_blk_gather");
+        k_blk_pad<<<grid_for((int64_t)nblk * 4), kBlock, 0, s>>>(bptr, boff, nblk, g->b_src,
+                                                                 g->b_dst, g->b_w);
+        SSSP_LAUNCH_CHECK("k_blk_pad");
         SSSP_HIP(hipStreamSynchronize(s));
     }
 

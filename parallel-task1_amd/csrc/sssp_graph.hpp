@@ -57,6 +57,18 @@ struct Ctrl {
     int32_t lb_n;             // queue entries plus pile entries moved onto the queue
 };
 
+// Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
+// block's run padded to a multiple of 4 edges with weight-0 self loops on the
+// block's first vertex. A tile is a run of at most kBlkTileEdges edges of one
+// block; `split` = the block has more than one tile.
+constexpr int kBlkBits = 14;
+constexpr int64_t kBlkTileEdges = (int64_t)1 << 19;
+struct BlkTile {
+    int32_t blk;
+    int32_t split;
+    int64_t e0, e1;
+};
+
 struct TraceRow {
     int32_t mode;
     int32_t chunk;
@@ -92,6 +104,12 @@ struct sssp_graph {
     uint32_t *x_w = nullptr;
     int64_t *xgoff = nullptr;
     std::vector<int64_t> xgoff_host;
+    // destination-blocked edge list (dense all-edge rounds)
+    int32_t *b_src = nullptr;
+    int32_t *b_dst = nullptr;
+    uint32_t *b_w = nullptr;
+    sssp::BlkTile *b_tile = nullptr;
+    int32_t n_btile = 0;
     // CSC: in-edges grouped by dst
     int64_t *col_ptr = nullptr;
     int32_t *csc_src = nullptr;
@@ -146,8 +164,9 @@ struct sssp_graph {
     double delta = 0.0;            // push rounds defer vertices with d >= min + delta (0 = off)
     int hub_degree = 0;            // ... unless their out-degree is below this
     int64_t plain_below_edges = (int64_t)1 << 25;  // heuristic/learned: plain passes below this E
-    double dense_all_frac = 0.7;  // all-edge rounds with frontier edges >= this * E run dense,
+    double dense_all_frac = 0.25;  // all-edge rounds with frontier edges >= this * E run dense,
     double dense_unreached = 0.5;  // ... and so do those with this sampled unreached fraction
+    int dense_kernel = 0;  // dense rounds: 0 = destination-blocked pass, 1 = pass + compaction
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds
     int trace_events = 0;
