@@ -12,11 +12,14 @@ Every source is solved with the learned, heuristic and plain schedules outside
 the timed region; the three distance arrays must be bit-identical and the
 learned one must pass the GPU certificate.
 
-HBM traffic of the all-edge pass (roofline.traffic): before this process
+roofline: the learned schedule's dominant kernel, the dense all-edge round
+(k_blk_pass), timed live with HIP events around each of its launches in one
+learned solve; plain_pass_roofline: the plain all-edge pass (k_el_pass) over
+one plain solve. HBM traffic per launch (traffic): before this process
 touches the GPU it runs itself twice under `rocprofv3 --pmc` (FETCH_SIZE, then
 WRITE_SIZE) in --pmc-child mode, which copies 1 GiB device to device and runs
-one plain solve; each counter is scaled by its value for the copy.
---traffic off skips this (traffic = null).
+one plain and one learned solve; each counter is scaled by its value for the
+copy. --traffic off skips this (traffic = null).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--traffic auto|off]
 """
@@ -108,12 +111,14 @@ def cpu_baseline(src, dst, w, V, source, plain_passes, max_passes, threads):
 
 COPY_BYTES = 1 << 30
 PASS_KERNEL = "k_el_pass"
+DENSE_KERNEL = "k_blk_pass"
 
 
 def pmc_child(args):
     """Profiled child: a 1 GiB device-to-device copy (calibration), then one
-    plain solve of the bench graph from the first bench source; prints the
-    pass count."""
+    plain solve and one learned solve of the bench graph from the first bench
+    source; prints the plain solve's pass count and the learned solve's dense
+    rounds (k_blk_pass launches)."""
     hip = C.CDLL("libamdhip64.so")
     a, b = C.c_void_p(), C.c_void_p()
     for p in (a, b):
@@ -125,10 +130,17 @@ def pmc_child(args):
         raise RuntimeError("calibration copy failed")
     hip.hipFree(a)
     hip.hipFree(b)
-    g = S.DeviceGraph.rmat(args.scale, args.edge_factor, seed=args.seed, threads=64, csc=False)
+    g = S.DeviceGraph.rmat(args.scale, args.edge_factor, seed=args.seed, threads=64)
     source = bench_sources(g, args)[0]
     _, st = g.solve(int(source), "plain", download=False)
-    print(json.dumps({"passes": int(st["iterations"]), "source": int(source)}), flush=True)
+    if os.path.exists(WEIGHTS):
+        mlp = S.MLP(0)
+        mlp.set_params(np.fromfile(WEIGHTS, np.float32))
+        g.set_mlp(mlp)
+    g.solve(int(source), "learned", download=False)
+    dense = int(np.isin(g.trace()["mode"], (S.MODE_VERIFY, S.MODE_DENSE)).sum())
+    print(json.dumps({"passes": int(st["iterations"]), "dense_rounds": dense,
+                      "source": int(source)}), flush=True)
 
 
 def bench_sources(g, args):
@@ -156,14 +168,15 @@ def _pmc_run(counter, args, workdir):
         raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
     if p.returncode != 0:
         raise RuntimeError(f"rocprofv3 --pmc {counter} exited {p.returncode}: {se[-400:]}")
-    passes = None
+    child = None
     for line in so.splitlines():
         if line.startswith("{"):
-            passes = json.loads(line)["passes"]
+            child = json.loads(line)
     files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
-    if passes is None or not files:
+    if child is None or not files:
         raise RuntimeError(f"rocprofv3 --pmc {counter}: no output")
-    copy_kib, pass_kib = [], []
+    passes, dense = child["passes"], child["dense_rounds"]
+    copy_kib, pass_kib, dense_kib = [], [], []
     with open(files[0]) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] != counter:
@@ -173,13 +186,17 @@ def _pmc_run(counter, args, workdir):
                 copy_kib.append(v)
             elif PASS_KERNEL in name:
                 pass_kib.append(v)
-    if not copy_kib or len(pass_kib) < passes:
+            elif DENSE_KERNEL in name:
+                dense_kib.append(v)
+    if not copy_kib or len(pass_kib) < passes or len(dense_kib) != dense:
         raise RuntimeError(f"rocprofv3 --pmc {counter}: dispatches missing")
-    # the copy's counter in KiB against the bytes it moves; the solve's first
-    # `passes` pass dispatches are the real passes (later queued ones return)
+    # the copy's counter in KiB against the bytes it moves; the plain solve's
+    # first `passes` pass dispatches are its real passes (later queued ones
+    # return at once); every dense-kernel dispatch is a dense round
     factor = COPY_BYTES / (max(copy_kib) * 1024.0)
     per_pass = float(np.mean(pass_kib[:passes])) * 1024.0 * factor
-    return per_pass, factor, passes
+    per_dense = float(np.mean(dense_kib)) * 1024.0 * factor if dense else None
+    return per_pass, per_dense, factor, passes, dense
 
 
 def copy_bandwidth(reps=5):
@@ -212,12 +229,18 @@ def copy_bandwidth(reps=5):
 
 
 def measure_traffic(args):
-    """HBM bytes per all-edge pass (FETCH + WRITE, copy-calibrated)."""
+    """HBM bytes per launch (FETCH + WRITE, copy-calibrated) of the plain
+    all-edge pass and of the dense round kernel."""
     with tempfile.TemporaryDirectory(dir="/tmp") as td:
-        fetch, ffac, passes = _pmc_run("FETCH_SIZE", args, td)
-        write, wfac, _ = _pmc_run("WRITE_SIZE", args, td)
-    return {"bytes_per_pass": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
-            "fetch_factor": ffac, "write_factor": wfac, "passes": passes}
+        fetch, dfetch, ffac, passes, dense = _pmc_run("FETCH_SIZE", args, td)
+        write, dwrite, wfac, _, _ = _pmc_run("WRITE_SIZE", args, td)
+    out = {"plain_pass": {"bytes_per_launch": fetch + write, "fetch_bytes": fetch,
+                          "write_bytes": write, "launches": passes},
+           "dense_round": None, "fetch_factor": ffac, "write_factor": wfac}
+    if dfetch is not None and dwrite is not None:
+        out["dense_round"] = {"bytes_per_launch": dfetch + dwrite, "fetch_bytes": dfetch,
+                              "write_bytes": dwrite, "launches": dense}
+    return out
 
 
 def main():
@@ -229,7 +252,7 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--seed", type=int, default=6)
-    ap.add_argument("--cpu-passes", type=int, default=3)
+    ap.add_argument("--cpu-passes", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--traffic", choices=("auto", "off"), default="auto")
@@ -307,26 +330,49 @@ def main():
         }
         log(sched, json.dumps(results[sched]))
 
-    # roofline of the all-edge pass kernel: per-pass HIP events on the stream
-    # the kernel runs on, over a plain solve of the first source
+    # rooflines, from HIP events recorded around each round on the graph's
+    # stream: the plain all-edge pass (k_el_pass) over a plain solve of the
+    # first source, and the learned schedule's dominant kernel, the dense
+    # all-edge round (k_blk_pass: one launch per dense round and per fixpoint
+    # check, nothing else inside its events), over a learned solve of the
+    # same source. Both stream 12 B per edge and read and write d once:
+    # 12 E + 8 V algorithmic bytes per launch.
+    algo_bytes = 12 * E + 8 * V
     g.set_param("trace_events", 1)
     _, st = g.solve(int(sources[0]), "plain", download=False)
     tr = g.trace()
+    g.solve(int(sources[0]), "learned", download=False)
+    trl = g.trace()
     g.set_param("trace_events", 0)
     pass_ms = float(np.mean(tr["ms"]))
-    algo_bytes = 12 * E + 8 * V
-    achieved = algo_bytes / (pass_ms * 1e-3) / 1e9
+    dense_ms_all = trl["ms"][np.isin(trl["mode"], (S.MODE_VERIFY, S.MODE_DENSE))]
+    dense_ms = float(np.mean(dense_ms_all))
     copy_gbs = copy_bandwidth()
     log(f"device-to-device copy: {copy_gbs} GB/s")
+    dense_t = traffic["dense_round"] if traffic else None
+    plain_t = traffic["plain_pass"] if traffic else None
+    achieved = algo_bytes / (dense_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm",
-                "kernel": "k_el_pass (all-edge pass, XCD-partitioned src-sorted edge list)",
+                "kernel": "k_blk_pass (dense all-edge round, destination-blocked edge list)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": round(traffic["bytes_per_pass"]) if traffic else None,
-                "traffic_detail": traffic,
-                "algorithmic_bytes_per_pass": algo_bytes, "avg_pass_ms": round(pass_ms, 4),
-                "achievable_copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
-                "passes": int(len(tr["ms"]))}
+                "traffic": round(dense_t["bytes_per_launch"]) if dense_t else None,
+                "traffic_detail": dense_t,
+                "algorithmic_bytes_per_launch": algo_bytes,
+                "avg_launch_ms": round(dense_ms, 4), "launches": int(len(dense_ms_all)),
+                "achievable_copy_GBs": round(copy_gbs, 1) if copy_gbs else None}
+    achieved_p = algo_bytes / (pass_ms * 1e-3) / 1e9
+    plain_roofline = {"bound": "hbm",
+                      "kernel": "k_el_pass (all-edge pass, XCD-partitioned src-sorted edge list)",
+                      "achieved": round(achieved_p, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(achieved_p / HBM_PEAK_GBS, 4),
+                      "traffic": round(plain_t["bytes_per_launch"]) if plain_t else None,
+                      "traffic_detail": plain_t,
+                      "algorithmic_bytes_per_launch": algo_bytes,
+                      "avg_launch_ms": round(pass_ms, 4), "launches": int(len(tr["ms"]))}
+    if traffic:
+        roofline["calibration"] = {"fetch_factor": traffic["fetch_factor"],
+                                   "write_factor": traffic["write_factor"]}
 
     cpu = None
     if not args.no_cpu:
@@ -361,6 +407,7 @@ def main():
         "scheduler_network": "loaded" if mlp else "absent",
         "schedules": results,
         "roofline": roofline,
+        "plain_pass_roofline": plain_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -56,9 +56,11 @@ extern "C" {
 #define SSSP_SCHED_PULL      5  /* every round: pull into unsettled vertices */
 
 /* ---- round modes reported in traces -------------------------------------- */
-#define SSSP_MODE_PUSH 0
-#define SSSP_MODE_ALL  1
-#define SSSP_MODE_PULL 2
+#define SSSP_MODE_PUSH  0
+#define SSSP_MODE_ALL   1  /* all-edge round with per-edge frontier appends */
+#define SSSP_MODE_PULL  2
+#define SSSP_MODE_CHECK 3  /* trace only: the final fixpoint check (run dense) */
+#define SSSP_MODE_DENSE 4  /* trace only: an all-edge round run dense (see dense_all_frac) */
 
 /* ---- graph creation flags ------------------------------------------------ */
 #define SSSP_INPUT_DEVICE 1u   /* src/dst/w are device pointers on `device` */

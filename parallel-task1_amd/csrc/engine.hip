@@ -1305,6 +1305,7 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
     const int32_t pn = h.pile_n[p];
     const bool snap_ok = r > 0 && g->snap_round == r - 1;
     g->snap_round = -1;  // any round but a dense one changes distances behind the snapshot
+    g->last_round_dense = false;
     if (mode != SSSP_MODE_PUSH && !verify && pn > 0) {
         k_pile_flush<<<grid_for(pn, 4096), kBlock, 0, s>>>(g->pile[p], pn, g->stamp, a.tick_prev,
                                                            g->farst, g->ctrl, p);
@@ -1323,6 +1324,7 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
             // global atomics, the frontier written by the same kernel), or
             // the plain pass followed by a compaction of the lowered vertices
             // against a snapshot
+            g->last_round_dense = true;
             if (g->dense_kernel == 0 && g->n_btile > 0) {
                 k_blk_pass<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, s>>>(
                     g->b_tile, g->b_src, g->b_dst, g->b_w, g->dist, g->V, g->stamp, a.tick_cur,
@@ -1560,7 +1562,7 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
         SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
     }
     TraceRow tr{};
-    tr.mode = verify ? 3 : mode;
+    tr.mode = verify ? SSSP_MODE_CHECK : g->last_round_dense ? SSSP_MODE_DENSE : mode;
     tr.chunk = kidx;
     tr.frontier = (int64_t)cur.qlen + piled;
     tr.frontier_edges = verify ? g->E : cur.qedges + piled_edges;

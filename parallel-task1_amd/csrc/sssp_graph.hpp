@@ -167,6 +167,7 @@ struct sssp_graph {
     double dense_all_frac = 0.25;  // all-edge rounds with frontier edges >= this * E run dense,
     double dense_unreached = 0.5;  // ... and so do those with this sampled unreached fraction
     int dense_kernel = 0;  // dense rounds: 0 = destination-blocked pass, 1 = pass + compaction
+    bool last_round_dense = false;  // the last round ran dense (traces)
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds
     int trace_events = 0;
