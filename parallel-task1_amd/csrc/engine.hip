@@ -1039,11 +1039,12 @@ constexpr int kBlkStageCap = 128;  // per-wave LDS staging slots
 template <class WT>
 __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
     const BlkTile *__restrict__ tiles, const int32_t *__restrict__ bsrc,
-    const int32_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
+    const uint16_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
     int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, int32_t *qnext, RoundSlot *slot)
 {
     typedef int32_t v4i __attribute__((ext_vector_type(4)));
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    typedef uint16_t v4h __attribute__((ext_vector_type(4)));
     constexpr int NB = 1 << kBlkBits;
     constexpr int PER = NB / kBlkThreads;  // block vertices per thread
     constexpr uint32_t INF = WT::INF;
@@ -1062,18 +1063,17 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
     }
     __syncthreads();
     const v4i *s4 = reinterpret_cast<const v4i *>(bsrc);
-    const v4i *t4 = reinterpret_cast<const v4i *>(bdst);
+    const v4h *t4 = reinterpret_cast<const v4h *>(bdst);
     const v4u *w4 = reinterpret_cast<const v4u *>(bw);
-    auto relax = [&](uint32_t du, int32_t v, uint32_t ww) {
+    auto relax = [&](uint32_t du, uint32_t loc, uint32_t ww) {
         if (du == INF) return;
         acc.relaxed++;
         const uint32_t cand = WT::add(du, ww);
-        const int loc = (int)(v - base);
         if (cand < s_d[loc]) atomicMin(&s_d[loc], cand);
     };
     for (int64_t g = (t.e0 >> 2) + threadIdx.x; g < (t.e1 >> 2); g += kBlkThreads) {
         const v4i sv = __builtin_nontemporal_load(&s4[g]);
-        const v4i tv = __builtin_nontemporal_load(&t4[g]);
+        const v4h tv = __builtin_nontemporal_load(&t4[g]);
         const v4u wv = __builtin_nontemporal_load(&w4[g]);
         // src-sorted inside the block: equal sources are loaded once
         const uint32_t u0 = dist[sv.x];

@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_gather(const uint32_t *skeys, co
                                                        int64_t E, const int64_t *bptr,
                                                        const int64_t *boff, const int32_t *src,
                                                        const int32_t *dst, const uint32_t *w,
-                                                       int32_t *bs, int32_t *bd, uint32_t *bw)
+                                                       int32_t *bs, uint16_t *bd, uint32_t *bw)
 {
     for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < E;
          j += (int64_t)gridDim.x * kBlock) {
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_gather(const uint32_t *skeys, co
         const int64_t o = boff[b] + (j - bptr[b]);
         const uint32_t p = perm[j];
         bs[o] = src[p];
-        bd[o] = dst[p];
+        bd[o] = (uint16_t)(dst[p] - (int32_t)(b << kBlkBits));
         bw[o] = w[p];
     }
 }
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_gather(const uint32_t *skeys, co
 // Up to 3 padding slots per block: weight-0 self loops on the block's first
 // vertex (a candidate equal to the distance it would lower: no change).
 __global__ __launch_bounds__(kBlock) void k_blk_pad(const int64_t *bptr, const int64_t *boff,
-                                                    int32_t nblk, int32_t *bs, int32_t *bd,
+                                                    int32_t nblk, int32_t *bs, uint16_t *bd,
                                                     uint32_t *bw)
 {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_blk_pad(const int64_t *bptr, const i
     const int64_t o = boff[b] + (bptr[b + 1] - bptr[b]) + (i & 3);
     if (o < boff[b + 1]) {
         bs[o] = b << kBlkBits;
-        bd[o] = b << kBlkBits;
+        bd[o] = 0;
         bw[o] = 0u;
     }
 }
@@ -619,7 +619,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         const int64_t Ep = bo[(size_t)nblk];
         SSSP_HIP(hipMemcpyAsync(boff, bo.data(), 8 * bo.size(), hipMemcpyHostToDevice, s));
         g->b_src = dmalloc<int32_t>((size_t)Ep);
-        g->b_dst = dmalloc<int32_t>((size_t)Ep);
+        g->b_dst = dmalloc<uint16_t>((size_t)Ep);
         g->b_w = dmalloc<uint32_t>((size_t)Ep);
         g->n_btile = (int32_t)tiles.size();
         g->b_tile = dmalloc<BlkTile>(tiles.size());

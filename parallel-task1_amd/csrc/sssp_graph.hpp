@@ -59,9 +59,11 @@ struct Ctrl {
 
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
 // block's run padded to a multiple of 4 edges with weight-0 self loops on the
-// block's first vertex. A tile is a run of at most kBlkTileEdges edges of one
+// block's first vertex; destinations are stored as 16-bit offsets in their
+// block. A tile is a run of at most kBlkTileEdges edges of one
 // block; `split` = the block has more than one tile.
 constexpr int kBlkBits = 14;
+static_assert(kBlkBits <= 16, "block offsets are stored in 16 bits");
 constexpr int64_t kBlkTileEdges = (int64_t)1 << 19;
 struct BlkTile {
     int32_t blk;
@@ -106,7 +108,7 @@ struct sssp_graph {
     std::vector<int64_t> xgoff_host;
     // destination-blocked edge list (dense all-edge rounds)
     int32_t *b_src = nullptr;
-    int32_t *b_dst = nullptr;
+    uint16_t *b_dst = nullptr;  // destination - block base
     uint32_t *b_w = nullptr;
     sssp::BlkTile *b_tile = nullptr;
     int32_t n_btile = 0;
