@@ -1065,21 +1065,21 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
     const v4i *s4 = reinterpret_cast<const v4i *>(bsrc);
     const v4h *t4 = reinterpret_cast<const v4h *>(bdst);
     const v4u *w4 = reinterpret_cast<const v4u *>(bw);
+    // branch-free: an unreached source gives INF, which lowers nothing; the
+    // LDS minimum returns nothing, so no lane waits on it
     auto relax = [&](uint32_t du, uint32_t loc, uint32_t ww) {
-        if (du == INF) return;
-        acc.relaxed++;
-        const uint32_t cand = WT::add(du, ww);
-        if (cand < s_d[loc]) atomicMin(&s_d[loc], cand);
+        const bool fin = du != INF;
+        acc.relaxed += fin ? 1 : 0;
+        atomicMin(&s_d[loc], fin ? WT::add(du, ww) : INF);
     };
     for (int64_t g = (t.e0 >> 2) + threadIdx.x; g < (t.e1 >> 2); g += kBlkThreads) {
         const v4i sv = __builtin_nontemporal_load(&s4[g]);
         const v4h tv = __builtin_nontemporal_load(&t4[g]);
         const v4u wv = __builtin_nontemporal_load(&w4[g]);
-        // src-sorted inside the block: equal sources are loaded once
-        const uint32_t u0 = dist[sv.x];
-        const uint32_t u1 = sv.y == sv.x ? u0 : dist[sv.y];
-        const uint32_t u2 = sv.z == sv.y ? u1 : dist[sv.z];
-        const uint32_t u3 = sv.w == sv.z ? u2 : dist[sv.w];
+        // the four gathers are issued together (equal sources, frequent in
+        // the src-sorted runs, hit the same line); a reuse test between them
+        // would serialise them behind one another's latency
+        const uint32_t u0 = dist[sv.x], u1 = dist[sv.y], u2 = dist[sv.z], u3 = dist[sv.w];
         relax(u0, tv.x, wv.x);
         relax(u1, tv.y, wv.y);
         relax(u2, tv.z, wv.z);
