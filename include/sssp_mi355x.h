@@ -163,7 +163,7 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
  * "dense_all_frac", "dense_unreached", "trace_events"; "cu_count" read-only).
- * dense_all_frac (default 0.25) and dense_unreached (default 0.5): an
+ * dense_all_frac (default 0.15) and dense_unreached (default 0.5): an
  * all-edge round whose frontier has at least dense_all_frac * E out-edges, or
  * that starts with at least that fraction of a fixed vertex sample unreached,
  * runs dense -- the plain pass over every reached source, then the lowered
