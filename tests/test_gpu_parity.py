@@ -414,13 +414,16 @@ def test_small_graphs_hand_the_rule_and_network_to_plain_passes(gpu, oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("frac,unreached", [(0.0, 2.0), (1e9, 0.0), (1e9, 2.0), (0.7, 0.5)])
-def test_dense_all_edge_rounds_keep_the_fixpoint(gpu, oracle, frac, unreached):
-    """All-edge rounds run dense (plain pass, then the lowered vertices are
-    compacted against a snapshot into the next queue) when the frontier is
-    large or most of the graph is unreached; forced on for every all-edge
-    round, on only by the unreached rule, and off (the fixpoint check stays
-    dense), the distances, levels and frontiers stay exact."""
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("frac,unreached", [(0.0, 2.0), (1e9, 0.0), (1e9, 2.0), (0.15, 0.5)])
+def test_dense_all_edge_rounds_keep_the_fixpoint(gpu, oracle, frac, unreached, kernel):
+    """All-edge rounds run dense when the frontier is large or most of the
+    graph is unreached: over the destination-blocked edge list (kernel 0: LDS
+    minima, the frontier written by the same kernel) or as the plain pass
+    followed by a compaction against a snapshot (kernel 1). Forced on for
+    every all-edge round, on only by the unreached rule, and off (the
+    fixpoint check stays dense), the distances, levels and frontiers stay
+    exact."""
     src, dst, w = S.rmat(14, 16, seed=41, threads=16)
     V = 1 << 14
     su, tu, wu = S.er(1 << 13, 8, seed=13, weight_mode=1)
@@ -429,6 +432,7 @@ def test_dense_all_edge_rounds_keep_the_fixpoint(gpu, oracle, frac, unreached):
             h.set_param("plain_below_edges", 0)  # exercise the round engine
             h.set_param("dense_all_frac", frac)
             h.set_param("dense_unreached", unreached)
+            h.set_param("dense_kernel", kernel)
             assert h.get_param("dense_all_frac") == frac
         for s in (0, int(src[77])):
             ref = oracle.dijkstra(src, dst, w, V, s)
@@ -445,3 +449,34 @@ def test_dense_all_edge_rounds_keep_the_fixpoint(gpu, oracle, frac, unreached):
             assert np.array_equal(d, oracle.dijkstra(su, tu, wu, 1 << 13, 0)), sched
         with pytest.raises(S.SSSPError):
             g.set_param("dense_unreached", -1)
+        with pytest.raises(S.SSSPError):
+            g.set_param("dense_kernel", 2)
+
+
+@pytest.mark.gpu
+def test_dense_rounds_with_split_blocks(gpu, oracle):
+    """A destination block with more in-edges than one tile holds is split
+    over several tiles, which write back with atomicMin and stamp with an
+    exchange; the R-MAT hub block here is split (checked on the host) and
+    every dense round stays exact, for f32, u32 and BFS levels."""
+    scale = 17
+    src, dst, w = S.rmat(scale, 16, seed=44, threads=16)
+    V = 1 << scale
+    per_block = np.bincount(dst >> 14)
+    assert per_block.max() > (1 << 19), per_block.max()  # at least one split block
+    wu = (np.arange(len(src)) % 255 + 1).astype(np.uint32)
+    with S.DeviceGraph(src, dst, w, V) as g, S.DeviceGraph(src, dst, wu, V) as gu:
+        for h in (g, gu):
+            h.set_param("plain_below_edges", 0)
+            h.set_param("dense_all_frac", 0.0)  # every all-edge round dense
+        for s in (0, int(src[999])):
+            ref = oracle.dijkstra(src, dst, w, V, s)
+            for sched in ("all", "heuristic", "learned"):
+                d, st = g.solve(s, sched)
+                assert np.array_equal(bits(d), bits(ref)), (s, sched)
+            ok, counts, _ = g.certify(s)
+            assert ok, counts
+            du, _ = gu.solve(s, "all")
+            assert np.array_equal(du, oracle.dijkstra(src, dst, wu, V, s)), s
+            _, lev, _ = g.bfs(s, "all")
+            assert np.array_equal(lev, oracle.bfs(src, dst, V, s)[1]), s
