@@ -269,8 +269,9 @@ def main():
         pmc_child(args)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched "
-              f"sources of this path would shard across GPUs, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): no multi-GPU "
+              f"path is built; the independent sources of a batch would be the unit to shard "
+              f"across GPUs, each holding a replica of the graph", file=sys.stderr)
         sys.exit(2)
 
     # host side only so far: the profiled children run before this process
