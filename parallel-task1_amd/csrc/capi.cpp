@@ -172,9 +172,14 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     SSSP_HIP(hipEventCreate(&g->ev_r0));
     SSSP_HIP(hipEventCreate(&g->ev_r1));
     SSSP_HIP(hipStreamSynchronize(s));
+    // resident arrays: the three edge orders and the CSC, row offsets, the
+    // per-vertex arrays (dist, stamp, piles, pile state, snapshot, parent,
+    // queues, load-balancing entries, static entry lists)
     const int64_t Ei = g->E, Vi = g->V;
-    g->device_bytes = Ei * 12 + (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) +
-                      Vi * 4 * 10 + Vi * 16 + Vi * 16;
+    const int64_t xe = g->xgoff_host.empty() ? 0 : g->xgoff_host.back();
+    g->device_bytes = Ei * 12 + xe * 12 + g->b_edges * 10 + (int64_t)g->n_btile * 24 +
+                      (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) + Vi * 4 * 7 +
+                      (Vi + 1) * 4 * 3 + (Vi + 1) * 8 * 2 + Vi * 24;
 }
 
 struct DevBuf {

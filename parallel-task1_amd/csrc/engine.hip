@@ -116,26 +116,48 @@ __device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
 // A vertex lowered in this round joins the next frontier when it has
 // out-edges and no other lane appended it in this round (stamp). A plain read
 // of the stamp filters sinks (kSinkStamp) and vertices already appended, so
-// the exchange is paid about once per appended vertex.
-template <int DIR>
-__device__ __forceinline__ void append_frontier(bool low, int32_t v, const int64_t *row_ptr,
-                                                int32_t *stamp, int32_t tick_cur, int64_t &qedges,
-                                                int32_t *wbuf, int &wcnt)
+// the exchange is paid about once per appended vertex. This handles the N
+// edges a lane relaxed in one step of its loop: the stamp and degree loads of
+// all N are issued together, then the exchanges, so the protocol's latency is
+// paid once per step rather than once per edge (the stamps still land within
+// the step that lowered the vertex). Flushes the wave's staging buffer
+// whenever fewer than 64 slots are left. (The certificate BFS, DIR_BFS,
+// appends every vertex with out-edges it claimed, without stamps.)
+template <int DIR, int N>
+__device__ __forceinline__ void append_frontier_n(const bool (&low)[N], const int32_t (&v)[N],
+                                                  const int64_t *row_ptr, int32_t *stamp,
+                                                  int32_t tick_cur, int64_t &qedges,
+                                                  int32_t *wbuf, int &wcnt, int32_t *q,
+                                                  int32_t *qlen)
 {
-    bool app = false;
-    if (low) {
-        if constexpr (DIR == DIR_BFS) {
-            const int64_t deg = row_ptr[v + 1] - row_ptr[v];
-            app = deg > 0;
-            qedges += deg;
-        } else {
-            const int32_t sv = stamp[v];
-            if (sv != tick_cur && sv != kSinkStamp)
-                app = atomicExch(&stamp[v], tick_cur) != tick_cur;
-            if (app) qedges += row_ptr[v + 1] - row_ptr[v];
+    int32_t sv[N];
+    int64_t r0[N], r1[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        sv[k] = kSinkStamp;
+        r0[k] = r1[k] = 0;
+        if (low[k]) {
+            if constexpr (DIR != DIR_BFS) sv[k] = stamp[v[k]];
+            r0[k] = row_ptr[v[k]];
+            r1[k] = row_ptr[v[k] + 1];
         }
     }
-    stage_push(app, v, wbuf, wcnt);
+    bool app[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        app[k] = false;
+        if (low[k]) {
+            if constexpr (DIR == DIR_BFS) app[k] = r1[k] > r0[k];
+            else if (sv[k] != tick_cur && sv[k] != kSinkStamp)
+                app[k] = atomicExch(&stamp[v[k]], tick_cur) != tick_cur;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (app[k]) qedges += r1[k] - r0[k];
+        stage_push(app[k], v[k], wbuf, wcnt);
+        if (wcnt > kStageCap - 64) stage_flush(wbuf, wcnt, q, qlen);
+    }
 }
 
 // Lowers dist[v] to cand without returning the old value. The caller has read
@@ -278,6 +300,8 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
             const bool act = (DIR == DIR_PULL) ? sk[k] != 0u : sk[k] != INF;
             dk[k] = act ? a.dist[ok[k]] : INF;
         }
+        bool lowk[K];
+        int32_t vk[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int32_t rel = k * kBlock + tid;
@@ -328,9 +352,11 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                     }
                 }
             }
-            append_frontier<DIR>(low, v, a.row_ptr, a.stamp, a.tick_cur, acc.qedges, wbuf, wcnt);
-            if (wcnt > kStageCap - 64) stage_flush(wbuf, wcnt, a.qnext, &a.slot->qlen);
+            lowk[k] = low;
+            vk[k] = v;
         }
+        append_frontier_n<DIR, K>(lowk, vk, a.row_ptr, a.stamp, a.tick_cur, acc.qedges, wbuf,
+                                  wcnt, a.qnext, &a.slot->qlen);
         __syncthreads();
     }
     stage_flush(wbuf, wcnt, a.qnext, &a.slot->qlen);
@@ -413,24 +439,26 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
 #pragma unroll
             for (int i = 0; i < 4; i++) dv[i] = du[i] != INF ? dist[et[i]] : 0u;
         }
+        bool low[4];
+        int32_t vv[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            bool low = false;
-            int32_t v = 0;
+            low[i] = false;
+            vv[i] = 0;
             if (du[i] != INF) {
-                v = et[i];
+                vv[i] = et[i];
                 const uint32_t cand = WT::add(du[i], ew[i]);
                 acc.relaxed++;
-                if (cand < dv[i] && filt_send<kFiltBitsRound>(s_filt, v, cand)) {
-                    lower_dist(dist, v, cand);
-                    low = true;
+                if (cand < dv[i] && filt_send<kFiltBitsRound>(s_filt, vv[i], cand)) {
+                    lower_dist(dist, vv[i], cand);
+                    low[i] = true;
                     acc.minw = min(acc.minw, cand);
                     acc.lowered++;
                 }
             }
-            append_frontier<DIR_PUSH>(low, v, row_ptr, stamp, tick_cur, acc.qedges, wbuf, wcnt);
         }
-        if (wcnt > kStageCap - 256) stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+        append_frontier_n<DIR_PUSH, 4>(low, vv, row_ptr, stamp, tick_cur, acc.qedges, wbuf, wcnt,
+                                       qnext, &slot->qlen);
     }
     stage_flush(wbuf, wcnt, qnext, &slot->qlen);
     flush_acc(acc, slot);

@@ -622,6 +622,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         g->b_dst = dmalloc<uint16_t>((size_t)Ep);
         g->b_w = dmalloc<uint32_t>((size_t)Ep);
         g->n_btile = (int32_t)tiles.size();
+        g->b_edges = Ep;
         g->b_tile = dmalloc<BlkTile>(tiles.size());
         SSSP_HIP(hipMemcpyAsync(g->b_tile, tiles.data(), sizeof(BlkTile) * tiles.size(),
                                 hipMemcpyHostToDevice, s));

@@ -112,6 +112,7 @@ struct sssp_graph {
     uint32_t *b_w = nullptr;
     sssp::BlkTile *b_tile = nullptr;
     int32_t n_btile = 0;
+    int64_t b_edges = 0;  // edges in the blocked list, padding included
     // CSC: in-edges grouped by dst
     int64_t *col_ptr = nullptr;
     int32_t *csc_src = nullptr;
