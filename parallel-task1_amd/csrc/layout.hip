@@ -612,9 +612,12 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
                                         std::min(e0 + kBlkTileEdges, bo[(size_t)b + 1])});
             }
         }
-        // largest tiles first: the dispatch order, so the grid drains evenly
+        // smallest tiles first (the dispatch order): the high in-degree
+        // blocks then run last, reading the distances the rest of the sweep
+        // has already lowered; measured against largest first, descending
+        // and ascending block id, it needs the fewest and cheapest rounds
         std::stable_sort(tiles.begin(), tiles.end(), [](const BlkTile &a, const BlkTile &b) {
-            return a.e1 - a.e0 > b.e1 - b.e0;
+            return a.e1 - a.e0 < b.e1 - b.e0;
         });
         const int64_t Ep = bo[(size_t)nblk];
         SSSP_HIP(hipMemcpyAsync(boff, bo.data(), 8 * bo.size(), hipMemcpyHostToDevice, s));

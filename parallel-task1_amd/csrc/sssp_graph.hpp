@@ -64,7 +64,7 @@ struct Ctrl {
 // block; `split` = the block has more than one tile.
 constexpr int kBlkBits = 14;
 static_assert(kBlkBits <= 16, "block offsets are stored in 16 bits");
-constexpr int64_t kBlkTileEdges = (int64_t)1 << 19;
+constexpr int64_t kBlkTileEdges = (int64_t)1 << 17;
 struct BlkTile {
     int32_t blk;
     int32_t split;
