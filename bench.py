@@ -10,7 +10,10 @@ cycling over the 8 sources; value = steps * E / wall time of the timed steps
 
 Every source is solved with the learned, heuristic and plain schedules outside
 the timed region; the three distance arrays must be bit-identical and the
-learned one must pass the GPU certificate.
+learned one must pass the GPU certificate. Besides the three schedules, the
+line reports all_dense_gteps: every round an all-edge round run dense (plain
+Bellman-Ford over the destination-blocked edge list), which separates the
+dense round kernel's share of the learned/plain ratio from the schedule's.
 
 roofline: the learned schedule's dominant kernel, the dense all-edge round
 (k_blk_pass), timed live with HIP events around each of its launches in one
@@ -318,10 +321,18 @@ def main():
             certified &= bool(ok and same)
             log(f"source {int(s)}: certificate {ok} {counts.tolist()} identical {same}")
 
-    # timed runs
+    # timed runs; "all_dense" (every round an all-edge round run dense, i.e.
+    # plain Bellman-Ford over the destination-blocked list) separates what the
+    # dense round kernel gives from what the schedule gives
     results = {}
-    for sched in ("learned", "heuristic", "plain"):
-        wall, tdev, stats = time_schedule(g, sources, sched, args.steps, args.warmup)
+    for sched in ("learned", "heuristic", "plain", "all_dense"):
+        if sched == "all_dense":
+            frac = g.get_param("dense_all_frac")
+            g.set_param("dense_all_frac", 0.0)
+            wall, tdev, stats = time_schedule(g, sources, "all", args.steps, args.warmup)
+            g.set_param("dense_all_frac", frac)
+        else:
+            wall, tdev, stats = time_schedule(g, sources, sched, args.steps, args.warmup)
         results[sched] = {
             "gteps": args.steps * E / wall / 1e9,
             "gteps_device": args.steps * E / (sum(tdev) * 1e-3) / 1e9,
@@ -408,6 +419,7 @@ def main():
                    "edges": E, "sources": [int(s) for s in sources], "schedule": "learned",
                    "parallelism": "single GPU"},
         "plain_gteps": round(results["plain"]["gteps"], 3),
+        "all_dense_gteps": round(results["all_dense"]["gteps"], 3),
         "heuristic_gteps": round(results["heuristic"]["gteps"], 3),
         "speedup_vs_plain": round(learned["gteps"] / results["plain"]["gteps"], 3),
         "learned_vs_heuristic": round(learned["gteps"] / results["heuristic"]["gteps"], 3),
