@@ -204,7 +204,13 @@ int sssp_bfs_edges(const int32_t *d_src, const int32_t *d_dst, int32_t vertices,
 
 /* ---- batched solves (one persistent launch, replayed through a hipGraph) -- */
 /* Many sources on one graph. dist_out: n*V elements (host) or NULL.
- * policy 0 = autotuned, 1 = one workgroup per source, 2 = merged frontier. */
+ * policy 0 = autotuned (untuned: 3 for n >= 32, else 2), 1 = one workgroup
+ * per source, 2 = the sources solved one after another on the full chip,
+ * 3 = vertex-major all-edge passes: the distances of up to 256 sources are
+ * held per vertex as one row, each pass reads every edge once for all of
+ * them; the layout it needs (16 bytes per edge) is built on its first use,
+ * and 256 * V distances are kept on the device. stats->iterations = the
+ * policy run. */
 int sssp_solve_batch(sssp_graph *g, const int32_t *sources, int32_t n, int32_t policy,
                      void *dist_out, sssp_stats *stats);
 /* Many independent graphs: graph k has vertex_counts[k] vertices and edges

@@ -44,7 +44,7 @@ void graph_free(sssp_graph *g)
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
                     g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap,
-                    g->b_src, g->b_dst, g->b_w, g->b_tile};
+                    g->b_src, g->b_dst, g->b_w, g->b_tile, g->m_rec, g->m_tile, g->mdist};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {

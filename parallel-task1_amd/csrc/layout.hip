@@ -296,6 +296,27 @@ __global__ __launch_bounds__(kBlock) void k_blk_pad(const int64_t *bptr, const i
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_m_keys(const int32_t *dst, int64_t E, uint32_t *keys)
+{
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * kBlock)
+        keys[e] = (uint32_t)dst[e] >> kMBits;
+}
+
+// record j = the edge at sorted position j (perm: its position in the
+// src-sorted list, whose order is kept within a block)
+__global__ __launch_bounds__(kBlock) void k_m_gather(const uint32_t *skeys, const uint32_t *perm,
+                                                     int64_t E, const int32_t *src,
+                                                     const int32_t *dst, const uint32_t *w,
+                                                     MRec *out)
+{
+    for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < E;
+         j += (int64_t)gridDim.x * kBlock) {
+        const uint32_t p = perm[j];
+        out[j] = MRec{src[p], w[p], dst[p] - (int32_t)(skeys[j] << kMBits), 0};
+    }
+}
+
 // row_ptr[v] = first position whose sorted key is >= v; row_ptr[V] = E.
 __global__ __launch_bounds__(kBlock) void k_rowptr_from_sorted(const uint32_t *keys, int64_t E,
                                                                int32_t V, int64_t *row_ptr)
@@ -463,6 +484,55 @@ void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, u
     if (n <= 0) return;
     k_split_records<<<grid_for(n), kBlock, 0, s>>>(rec, n, src, dst, w);
     SSSP_LAUNCH_CHECK("k_split_records");
+}
+
+void build_multi_layout(sssp_graph *g)
+{
+    if (g->m_rec || g->E == 0) return;
+    hipStream_t s = g->stream;
+    const int64_t E = g->E;
+    const int32_t nblk = (int32_t)(((int64_t)g->V + (1 << kMBits) - 1) >> kMBits);
+    const int64_t rs_tiles = (E + kRsTile - 1) / kRsTile;
+    uint32_t *keys = dmalloc<uint32_t>((size_t)E);
+    uint32_t *kA = dmalloc<uint32_t>((size_t)E), *kB = dmalloc<uint32_t>((size_t)E);
+    uint32_t *vA = dmalloc<uint32_t>((size_t)E), *vB = dmalloc<uint32_t>((size_t)E);
+    int64_t *counts = dmalloc<int64_t>((size_t)std::max<int64_t>(256 * rs_tiles, (int64_t)nblk + 1));
+    int64_t *bptr = dmalloc<int64_t>((size_t)nblk + 1);
+    struct Scratch {
+        void *p[7];
+        ~Scratch()
+        {
+            for (void *x : p) dfree(x);
+        }
+    } scratch{{keys, kA, kB, vA, vB, counts, bptr}};
+    k_m_keys<<<grid_for(E), kBlock, 0, s>>>(g->el_dst, E, keys);
+    SSSP_LAUNCH_CHECK("k_m_keys");
+    uint32_t *skeys = nullptr, *perm = nullptr;
+    radix_sort_pairs(reinterpret_cast<const int32_t *>(keys), E, nblk - 1, kA, kB, vA, vB, counts,
+                     g->blk_a, g->blk_b, g->n_blk, s, &skeys, &perm);
+    MRec *rec = dmalloc<MRec>((size_t)E);
+    k_m_gather<<<grid_for(E), kBlock, 0, s>>>(skeys, perm, E, g->el_src, g->el_dst, g->el_w, rec);
+    SSSP_LAUNCH_CHECK("k_m_gather");
+    k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(skeys, E, nblk, bptr);
+    SSSP_LAUNCH_CHECK("k_rowptr_from_sorted multi");
+    std::vector<int64_t> bp((size_t)nblk + 1);
+    SSSP_HIP(hipMemcpyAsync(bp.data(), bptr, 8 * bp.size(), hipMemcpyDeviceToHost, s));
+    SSSP_HIP(hipStreamSynchronize(s));
+    std::vector<BlkTile> tiles;
+    for (int32_t b = 0; b < nblk; b++) {
+        const int64_t c = bp[(size_t)b + 1] - bp[(size_t)b];
+        const int64_t nt = (c + kMTileEdges - 1) / kMTileEdges;
+        for (int64_t k = 0; k < nt; k++) {
+            const int64_t e0 = bp[(size_t)b] + k * kMTileEdges;
+            tiles.push_back(BlkTile{b, nt > 1 ? 1 : 0, e0, std::min(e0 + kMTileEdges, bp[(size_t)b + 1])});
+        }
+    }
+    BlkTile *td = dmalloc<BlkTile>(std::max<size_t>(tiles.size(), 1));
+    SSSP_HIP(hipMemcpyAsync(td, tiles.data(), sizeof(BlkTile) * tiles.size(), hipMemcpyHostToDevice, s));
+    SSSP_HIP(hipStreamSynchronize(s));
+    g->m_rec = rec;
+    g->m_tile = td;
+    g->n_mtile = (int32_t)tiles.size();
 }
 
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,

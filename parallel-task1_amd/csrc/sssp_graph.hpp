@@ -71,6 +71,17 @@ struct BlkTile {
     int64_t e0, e1;
 };
 
+// Multi-source batches on one graph: edges as 16-byte records sorted by
+// (dst >> kMBits, src); tiles of at most kMTileEdges edges of one block.
+constexpr int kMBits = 6;
+constexpr int64_t kMTileEdges = 4096;
+struct MRec {
+    int32_t src;
+    uint32_t w;
+    int32_t loc;  // dst - block base
+    int32_t pad;
+};
+
 struct TraceRow {
     int32_t mode;
     int32_t chunk;
@@ -113,6 +124,13 @@ struct sssp_graph {
     sssp::BlkTile *b_tile = nullptr;
     int32_t n_btile = 0;
     int64_t b_edges = 0;  // edges in the blocked list, padding included
+    // multi-source batches (built on first use): records, tiles, and the
+    // vertex-major distances of up to 256 sources (D[v * n + s])
+    sssp::MRec *m_rec = nullptr;
+    sssp::BlkTile *m_tile = nullptr;
+    int32_t n_mtile = 0;
+    uint32_t *mdist = nullptr;
+    int64_t mdist_elems = 0;
     // CSC: in-edges grouped by dst
     int64_t *col_ptr = nullptr;
     int32_t *csc_src = nullptr;
@@ -246,6 +264,8 @@ void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, u
                    hipStream_t s);
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
                    bool build_csc);
+// the multi-source records and tiles (m_rec, m_tile), built once per graph
+void build_multi_layout(sssp_graph *g);
 void device_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *blk_a,
                                int64_t *blk_b, int64_t n_blk_cap, hipStream_t s);
 int64_t scan_blocks_needed(int64_t n);

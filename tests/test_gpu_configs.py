@@ -153,6 +153,9 @@ def test_c4a_batch_of_256_sources(gpu, oracle):
         D1, st1 = g.solve_batch(sources, policy=1)
         D2, st2 = g.solve_batch(sources, policy=2)
         assert np.array_equal(bits(D1), bits(D2))
+        g.solve_batch(sources, policy=3)  # builds the vertex-major layout once
+        D3, st3 = g.solve_batch(sources, policy=3)
+        assert np.array_equal(bits(D1), bits(D3))
         for i in range(256):
             d, _ = g.solve(int(sources[i]), "heuristic")
             assert np.array_equal(bits(D1[i]), bits(d)), i
@@ -162,7 +165,8 @@ def test_c4a_batch_of_256_sources(gpu, oracle):
         ok, counts, _ = g.certify(int(sources[255]), parents=False)
         assert ok, counts
         print("C4a", {"policy1": round(_gteps(256 * len(src), st1["t_solve_ms"]), 2),
-                      "policy2": round(_gteps(256 * len(src), st2["t_solve_ms"]), 2)}, "GTEPS")
+                      "policy2": round(_gteps(256 * len(src), st2["t_solve_ms"]), 2),
+                      "policy3": round(_gteps(256 * len(src), st3["t_solve_ms"]), 2)}, "GTEPS")
 
 
 @pytest.mark.gpu

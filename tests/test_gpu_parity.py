@@ -207,7 +207,7 @@ def test_device_rmat_generator_matches_host(gpu, scale, ef, seed, threads):
 
 # ---------------------------------------------------------------- batches
 @pytest.mark.gpu
-@pytest.mark.parametrize("policy", [1, 2])
+@pytest.mark.parametrize("policy", [1, 2, 3])
 def test_batch_sources_match_single_solves(gpu, oracle, policy):
     src, dst, w = S.rmat(13, 16, seed=21, threads=16)
     V = 1 << 13
@@ -237,6 +237,29 @@ def test_batch_u32(gpu, oracle):
         D, _ = g.solve_batch(np.array([0, 5, 9], np.int32), policy=1)
         for i, s in enumerate((0, 5, 9)):
             assert np.array_equal(D[i], oracle.dijkstra(src, dst, w, V, s))
+
+
+@pytest.mark.gpu
+def test_batch_vertex_major_chunks_and_u32(gpu, oracle):
+    """Policy 3 (vertex-major all-edge passes) in chunks of 256 sources: 300
+    sources on an R-MAT graph with split blocks (a hub block with more than
+    4096 in-edges), repeated sources, and integer weights."""
+    src, dst, w = S.rmat(12, 16, seed=23, threads=8)
+    V = 1 << 12
+    assert np.bincount(dst >> 6).max() > 4096  # at least one split block
+    sources = np.concatenate([S.pick_sources(src, V, 23, 290), np.zeros(10, np.int32)])
+    with S.DeviceGraph(src, dst, w, V) as g:
+        D, st = g.solve_batch(sources, policy=3)
+        assert D.shape == (300, V) and st["t_solve_ms"] > 0
+        D1, _ = g.solve_batch(sources, policy=1)
+        assert np.array_equal(bits(D), bits(D1))
+        for i in (0, 255, 256, 299):
+            assert np.array_equal(bits(D[i]), bits(oracle.dijkstra(src, dst, w, V, int(sources[i])))), i
+    su, tu, wu = S.er(1 << 12, 8, seed=4, weight_mode=1)
+    with S.DeviceGraph(su, tu, wu, 1 << 12) as g:
+        D, _ = g.solve_batch(np.array([0, 5, 9], np.int32), policy=3)
+        for i, s in enumerate((0, 5, 9)):
+            assert np.array_equal(D[i], oracle.dijkstra(su, tu, wu, 1 << 12, s))
 
 
 def _pack_case(n, seed):
