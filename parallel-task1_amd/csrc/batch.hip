@@ -14,11 +14,11 @@
 //   graph are solved one after another on the full chip.
 // Policy 3 (many sources on one graph, vertex-major): the distances of up to
 //   256 sources are one array D[v * n + s], so a vertex's distances are one
-//   contiguous row. All-edge passes over the edges sorted by 64-vertex
+//   contiguous row. All-edge passes over the edges sorted by 16-vertex
 //   destination block: a 256-thread workgroup (thread s = source s) holds its
-//   block's 64 x 256 distances in LDS, reads each edge record once for all its
+//   block's 16 x 256 distances in LDS, reads each edge record once for all its
 //   sources (a wave-uniform load), gathers the source vertex's row (coalesced,
-//   sixteen rows in flight)
+//   eight rows in flight)
 //   and lowers its own LDS column -- no atomics inside a block; the block is
 //   written back at the end (atomicMin for blocks split over several tiles).
 //   Passes run until one changes nothing, the plain Bellman-Ford fixpoint of
@@ -361,10 +361,12 @@ __global__ __launch_bounds__(kMThreads) void k_multi_pass(const BlkTile *__restr
         if (cand < ld[r.loc][s]) ld[r.loc][s] = cand;
     };
     int64_t e = t.e0;
-    // sixteen records at a time: their row gathers are in flight together
-    // (8: 23.0, 16: 25.8-27.4, 24: 26.0, 32: 24.1 GTEPS at R-MAT scale 20 x
-    // 256 sources; the records live in scalar registers)
-    constexpr int R = 16;
+    // eight records at a time: their row gathers are in flight together.
+    // Measured at R-MAT scale 20 x 256 sources (GTEPS): 64-vertex blocks (64
+    // KB of LDS, 2 waves per SIMD) 23.0 with 8 in flight, 25.8-27.4 with 16;
+    // 32-vertex blocks 44.0 (16); 16-vertex blocks (16 KB, 8 workgroups per
+    // CU) 51.4 (16), 54.7 (8); 8-vertex blocks 51.1 (16)
+    constexpr int R = 8;
     for (; e + R <= t.e1; e += R) {
         MRec r[R];
         uint32_t du[R];
@@ -544,7 +546,7 @@ Config default_config(sssp_graph *g, int32_t n)
     if (g->batch_policy)
         return Config{g->batch_policy, g->batch_bs, g->batch_wgpcu, g->push_edge_frac, g->push_chunk};
     // untuned: policy 3 for 32 or more sources on one graph (R-MAT scale 20
-    // x 256 sources: 26-27 GTEPS against 10.5 for policy 1), policy 2 below
+    // x 256 sources: 54.7 GTEPS against 10.5 for policy 1), policy 2 below
     if (n >= 32) return Config{3, kMThreads, 1, 0.0, 0};
     return Config{2, 256, 2, g->push_edge_frac, g->push_chunk};
 }

@@ -72,8 +72,10 @@ struct BlkTile {
 };
 
 // Multi-source batches on one graph: edges as 16-byte records sorted by
-// (dst >> kMBits, src); tiles of at most kMTileEdges edges of one block.
-constexpr int kMBits = 6;
+// (dst >> kMBits, src) -- 16-vertex destination blocks, so a 256-source block
+// of distances is 16 KB of LDS; tiles of at most kMTileEdges edges of one
+// block.
+constexpr int kMBits = 4;
 constexpr int64_t kMTileEdges = 4096;
 struct MRec {
     int32_t src;
