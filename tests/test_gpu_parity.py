@@ -299,8 +299,18 @@ def test_autotune_batch_and_pack(gpu, tmp_path):
         assert best > 0
         text = (tmp_path / "tune.txt").read_text()
         assert text.count("best") == 1 and "workgroup_per_task" in text and "merged_frontier" in text
+        assert "vertex_major_all_edge" in text
         D, _ = g.solve_batch(sources, policy=0)
         assert np.array_equal(bits(D), bits(ref))
+    # untuned: 32 or more sources on one graph run policy 3, fewer policy 2
+    many = S.pick_sources(src, V, 3, 40)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        D, st = g.solve_batch(many, policy=0)
+        assert st["iterations"] == 3
+        D2, _ = g.solve_batch(many, policy=1)
+        assert np.array_equal(bits(D), bits(D2))
+        _, st = g.solve_batch(many[:8], policy=0)
+        assert st["iterations"] == 2
     scales, s, d, w2, offs, vc, srcs = _pack_case(24, 5)
     p = S.GraphPack(s, d, w2, offs, vc)
     try:
