@@ -1086,10 +1086,14 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
     const BlkTile t = tiles[blockIdx.x];
     const int64_t base = (int64_t)t.blk << kBlkBits;
     const int nb = (int)min((int64_t)NB, V - base);
+    // the block's distances as loaded, kept in registers for the write-back
+    // (a second read of the block would cost 64 KB of HBM per tile)
+    uint32_t d0[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int i = k * kBlkThreads + threadIdx.x;
-        s_d[i] = i < nb ? dist[base + i] : INF;
+        d0[k] = i < nb ? dist[base + i] : INF;
+        s_d[i] = d0[k];
     }
     __syncthreads();
     const v4i *s4 = reinterpret_cast<const v4i *>(bsrc);
@@ -1122,10 +1126,11 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
         const int32_t v = (int32_t)(base + i);
         bool app = false;
         if (i < nb) {
-            // the block's distance now: the value loaded above unless another
-            // tile of a split block lowered it since
+            // below the loaded value: this tile lowered v. For a split block
+            // another tile may have lowered it further since; the returning
+            // atomicMin then reports no change.
             const uint32_t nv = s_d[i];
-            if (nv < dist[v]) {
+            if (nv < d0[k]) {
                 bool low = true;
                 if (t.split) low = atomicMin(&dist[v], nv) > nv;
                 else dist[v] = nv;  // only this workgroup writes the block
