@@ -75,7 +75,7 @@ typedef struct sssp_stats {
     int32_t n_push;          /* rounds executed as sparse push */
     int32_t n_all;           /* rounds executed as all-edge sweeps (incl. plain passes) */
     int32_t n_pull;          /* rounds executed as pull */
-    int32_t verify_passes;   /* full all-edge fixpoint checks (the "judge") */
+    int32_t verify_passes;   /* full all-edge fixpoint checks (must change nothing) */
     int32_t mlp_rounds;      /* rounds whose mode the MLP chose */
     int32_t reached;         /* vertices with a finite distance */
     int32_t reserved;
