@@ -162,13 +162,17 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
 /* Tuning knobs by name ("push_chunk", "all_chunk", "pull_chunk",
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
- * "dense_all_frac", "dense_unreached", "trace_events"; "cu_count" read-only).
- * dense_all_frac (default 0.15) and dense_unreached (default 0.5): an
- * all-edge round whose frontier has at least dense_all_frac * E out-edges, or
- * that starts with at least that fraction of a fixed vertex sample unreached,
- * runs dense -- the plain pass over every reached source, then the lowered
- * vertices are compacted against a snapshot of the distances into the next
- * frontier -- instead of appending per edge; the fixpoint check always does.
+ * "dense_all_frac", "dense_unreached", "dense_kernel", "trace_events"; "cu_count"
+ * read-only).
+ * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
+ * 2^14 destinations with twice the mean in-edges) and dense_unreached
+ * (default 0.5): an all-edge round whose frontier has at least
+ * dense_all_frac * E out-edges, or that starts with at least dense_unreached
+ * of a fixed vertex sample unreached, runs dense instead of appending per
+ * edge; the fixpoint check always does. dense_kernel 0 (default) runs dense
+ * rounds over the destination-blocked edge list (LDS minima per block, the
+ * lowered vertices appended by the same kernel); 1 runs the plain pass, then
+ * compacts the lowered vertices against a snapshot into the next frontier.
  * rollout_policy and collect_kind steer sssp_collect_timings: the rounds
  * after the tried one follow the rule (0) or the attached network (1); the
  * collected solve is an SSSP (0) or a BFS (1).

@@ -187,7 +187,8 @@ struct sssp_graph {
     double delta = 0.0;            // push rounds defer vertices with d >= min + delta (0 = off)
     int hub_degree = 0;            // ... unless their out-degree is below this
     int64_t plain_below_edges = (int64_t)1 << 25;  // heuristic/learned: plain passes below this E
-    double dense_all_frac = 0.15;  // all-edge rounds with frontier edges >= this * E run dense,
+    double dense_all_frac = 0.15;  // all-edge rounds with frontier edges >= this * E run dense
+                                   // (0.4 when the layout build finds no hub blocks),
     double dense_unreached = 0.5;  // ... and so do those with this sampled unreached fraction
     int dense_kernel = 0;  // dense rounds: 0 = destination-blocked pass, 1 = pass + compaction
     bool last_round_dense = false;  // the last round ran dense (traces)

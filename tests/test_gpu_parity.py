@@ -447,6 +447,26 @@ def test_small_graphs_hand_the_rule_and_network_to_plain_passes(gpu, oracle):
 
 
 @pytest.mark.gpu
+def test_dense_threshold_default_follows_the_graph(gpu, oracle):
+    """A graph without hub blocks (uniform random: every block of 2^14
+    destinations near the mean in-edge count) runs dense rounds from 0.4 E of
+    frontier edges, a skewed one (R-MAT) from 0.15 E; a set value stays."""
+    V = 1 << 16
+    src, dst, w = S.er(V, 16, seed=3, weight_mode=1)
+    ref = oracle.dijkstra(src, dst, w, V, 0)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        assert g.get_param("dense_all_frac") == 0.4
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
+        d, _ = g.solve(0, "heuristic")
+        assert np.array_equal(bits(d), bits(ref))
+        g.set_param("dense_all_frac", 0.2)
+        assert g.get_param("dense_all_frac") == 0.2
+    src, dst, w = S.rmat(18, 8, seed=3, threads=8)
+    with S.DeviceGraph(src, dst, w, 1 << 18) as g:
+        assert g.get_param("dense_all_frac") == 0.15
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kernel", [0, 1])
 @pytest.mark.parametrize("frac,unreached", [(0.0, 2.0), (1e9, 0.0), (1e9, 2.0), (0.15, 0.5)])
 def test_dense_all_edge_rounds_keep_the_fixpoint(gpu, oracle, frac, unreached, kernel):
