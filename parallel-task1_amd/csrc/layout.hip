@@ -680,15 +680,21 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         for (int32_t b = 0; b < nblk; b++)
             max_blk = std::max(max_blk, bp[(size_t)b + 1] - bp[(size_t)b]);
         if ((double)max_blk * nblk < 2.0 * (double)E) g->dense_all_frac = 0.4;
+        // Tile cap: 2^17 edges (fastest at 2^27-2^29 edges), doubled on larger
+        // graphs until about kBlkTileCount tiles cover them -- the 2.08 G-edge
+        // R-MAT graph ran 50.9 GTEPS with 2^19-edge tiles, 46.7 with 2^17.
+        // A power of two, so every tile starts on a 4-edge group.
+        int64_t tcap = kBlkTileEdges;
+        while (tcap * kBlkTileCount < E) tcap <<= 1;
         std::vector<BlkTile> tiles;
         for (int32_t b = 0; b < nblk; b++) {
             const int64_t c = (bp[(size_t)b + 1] - bp[(size_t)b] + 3) / 4 * 4;
             bo[(size_t)b + 1] = bo[(size_t)b] + c;
-            const int64_t nt = (c + kBlkTileEdges - 1) / kBlkTileEdges;
+            const int64_t nt = (c + tcap - 1) / tcap;
             for (int64_t k = 0; k < nt; k++) {
-                const int64_t e0 = bo[(size_t)b] + k * kBlkTileEdges;
+                const int64_t e0 = bo[(size_t)b] + k * tcap;
                 tiles.push_back(BlkTile{b, nt > 1 ? 1 : 0, e0,
-                                        std::min(e0 + kBlkTileEdges, bo[(size_t)b + 1])});
+                                        std::min(e0 + tcap, bo[(size_t)b + 1])});
             }
         }
         // smallest tiles first (the dispatch order): the high in-degree

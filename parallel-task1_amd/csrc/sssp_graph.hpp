@@ -60,11 +60,13 @@ struct Ctrl {
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
 // block's run padded to a multiple of 4 edges with weight-0 self loops on the
 // block's first vertex; destinations are stored as 16-bit offsets in their
-// block. A tile is a run of at most kBlkTileEdges edges of one
-// block; `split` = the block has more than one tile.
+// block. A tile is a run of at most kBlkTileEdges edges of one block (the
+// cap doubles until the graph has at most kBlkTileCount tiles' worth of
+// edges); `split` = the block has more than one tile.
 constexpr int kBlkBits = 14;
 static_assert(kBlkBits <= 16, "block offsets are stored in 16 bits");
 constexpr int64_t kBlkTileEdges = (int64_t)1 << 17;
+constexpr int64_t kBlkTileCount = 4096;
 struct BlkTile {
     int32_t blk;
     int32_t split;
