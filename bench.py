@@ -5,7 +5,8 @@ E = 536,870,912), a/b/c/d = 57/19/19/5, f32 weights float(rand_r)/RAND_MAX,
 base seed 6 with 64 virtual generator threads, 8 sources drawn among vertices
 with out-degree > 0 (std::mt19937_64(6)). Generated on the device
 (bit-identical to the host generator). One step = one single-source solve,
-cycling over the 8 sources; value = steps * E / wall time of the timed steps
+cycling over the 8 sources (the default 16 steps solve each twice); value =
+steps * E / wall time of the timed steps
 (GTEPS, the reference's E / t_solve convention, sssp.hpp:274-276).
 
 Every source is solved with the learned, heuristic and plain schedules outside
@@ -81,15 +82,23 @@ def time_schedule(g, sources, sched, steps, warmup):
     return wall, t_dev, stats
 
 
-def cpu_baseline(src, dst, w, V, source, plain_passes, max_passes, threads):
+def cpu_baseline(src, dst, w, V, source, plain_passes, max_passes):
     """The oracle's OpenMP restatement of cpu_bellman_ford on the same graph,
-    stopped after `max_passes` passes; GTEPS = E / (time per pass * passes the
-    full solve needs)."""
+    on omp_get_max_threads() threads as sample/sssp.cpp:82 sets them (OpenMP
+    honours OMP_NUM_THREADS and the process's CPU affinity), stopped after
+    `max_passes` passes; GTEPS = E / (time per pass * passes the full solve
+    needs)."""
     lib = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
     p = C.c_void_p
     f = lib.orc_bf_f32_capped
     f.argtypes = [p, p, p, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32, p, p]
     f.restype = C.c_int
+    lib.orc_max_threads.restype = C.c_int
+    threads = int(lib.orc_max_threads())
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
     d = np.empty(V, np.float32)
     conv = C.c_int32()
     P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
@@ -116,6 +125,11 @@ def cpu_baseline(src, dst, w, V, source, plain_passes, max_passes, threads):
         "cores": threads,
         "kind": "port",
         "sample": sample,
+        "omp_threads": threads,
+        "nproc": os.cpu_count(),
+        "affinity_cpus": affinity,
+        "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+        "cpu_model": cpu_model(),
     }
 
 
@@ -256,7 +270,7 @@ def measure_traffic(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=25)
     ap.add_argument("--edge-factor", type=int, default=16)
@@ -395,9 +409,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(h_src, h_dst, h_w, V, sources[0], int(st["iterations"]),
-                           args.cpu_passes, threads)
+                           args.cpu_passes)
         log("cpu", json.dumps(cpu))
 
     learned = results["learned"]

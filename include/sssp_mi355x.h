@@ -93,6 +93,13 @@ int sssp_version(void);
 int sssp_device_count(int32_t *count);
 int sssp_device_info(int32_t device, char *name, int32_t name_len, int32_t *cu_count,
                      int64_t *hbm_bytes, int32_t *l2_bytes);
+/* What NodeData collects per device (include/node_data/node_data.cpp:120-147:
+ * cudaGetDeviceProperties + cudaMemGetInfo): the device's product name, free
+ * and total global memory, the largest workgroup and the largest grid
+ * (max_grid: 3 values). Any output may be NULL. The calling thread's current
+ * device is the same before and after the call. */
+int sssp_device_props(int32_t device, char *name, int32_t name_len, int64_t *free_bytes,
+                      int64_t *total_bytes, int32_t *max_threads_per_block, int32_t *max_grid);
 
 /* ---- graph on the device ------------------------------------------------- */
 /* Uploads an edge list (SoA, as EdgesList / GraphContainerData hold it:
@@ -162,8 +169,8 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
 /* Tuning knobs by name ("push_chunk", "all_chunk", "pull_chunk",
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
- * "dense_all_frac", "dense_unreached", "dense_kernel", "trace_events"; "cu_count"
- * read-only).
+ * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
+ * "trace_events"; "cu_count" and "collect_rounds" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
  * 2^14 destinations with twice the mean in-edges) and dense_unreached
  * (default 0.5): an all-edge round whose frontier has at least
@@ -175,13 +182,19 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * compacts the lowered vertices against a snapshot into the next frontier.
  * rollout_policy and collect_kind steer sssp_collect_timings: the rounds
  * after the tried one follow the rule (0) or the attached network (1); the
- * collected solve is an SSSP (0) or a BFS (1).
+ * collected solve is an SSSP (0) or a BFS (1). collect_budget (default 2^22)
+ * bounds the rounds one collection runs in total (its cost grows with the
+ * square of the solve's round count); past it the call returns
+ * SSSP_ERR_NOT_CONVERGED. "collect_rounds" (read-only) = rounds the last
+ * collection ran.
  * plain_below_edges (default 2^25): a single-source heuristic or learned
  * solve (or BFS) of a graph with fewer edges runs the plain pass loop, whose
  * passes then cost less than a round's fixed overhead.
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with
  * d >= (min distance over the frontier and pile) + delta and out-degree >=
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.
+ * max_rounds (default 100000) is bounded to [1, 2^28]; a solve that needs more
+ * rounds returns SSSP_ERR_NOT_CONVERGED.
  * Distances do not depend on any knob. */
 int sssp_set_param(sssp_graph *g, const char *name, double value);
 int sssp_get_param(const sssp_graph *g, const char *name, double *value);
@@ -202,7 +215,8 @@ int sssp_bfs(sssp_graph *g, int32_t source, int32_t schedule, uint8_t *visited_o
 /* bfs_wrapper (algorithm/bfs/gpu/bfs.cu:44-85) over caller-owned device
  * arrays: d_visited (device, V bytes) = 1 for every vertex reachable from
  * source over the edges (d_src[i] -> d_dst[i]), after all-edge passes until
- * one marks nothing; *passes (may be NULL) = passes run. */
+ * one marks nothing; *passes (may be NULL) = passes run. Ids outside
+ * [0, vertices) return SSSP_ERR_ARG before any pass. */
 int sssp_bfs_edges(const int32_t *d_src, const int32_t *d_dst, int32_t vertices, int64_t edges,
                    uint8_t *d_visited, int32_t source, int32_t *passes);
 
@@ -243,7 +257,10 @@ int sssp_init_distances_u32(uint32_t *d_dist, int32_t vertices, int32_t source);
 /* bellman_ford_wrapper<W> (bellman_ford.cu:103-144): all-edge passes over the
  * caller's edge list, in its order, until a pass changes nothing (at most
  * `vertices` passes). Convergence is tested on the device; *passes (may be
- * NULL) receives the pass count. */
+ * NULL) receives the pass count. The edge arrays are checked on the device
+ * first: a vertex id outside [0, vertices) or a negative or NaN weight returns
+ * SSSP_ERR_ARG before any pass (the distances are compared as unsigned bit
+ * patterns, which order non-negative values only). */
 int sssp_bellman_ford_f32(const int32_t *d_src, const int32_t *d_dst, const float *d_w,
                           int32_t vertices, int64_t edges, int32_t source, float *d_dist,
                           int32_t *passes);

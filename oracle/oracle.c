@@ -21,10 +21,16 @@
  *                                (queue over the CSR, reachability from the
  *                                source), plus the hop level of every vertex
  *
+ *   orc_bf_f32_capped         -- orc_bf_f32 stopped after a pass cap (the
+ *                                bounded CPU baseline of the bench)
+ *
  * Pins (tests/test_oracle.py): the reference's own cpu_bellman_ford compiled
- * from /root/reference into oracle/_ref (golden vectors in tests/golden/),
- * Dijkstra bit-equality, brute force on tiny graphs, closed forms (chains,
- * stars, disconnected parts), edge-permutation invariance.
+ * from /root/reference into oracle/_ref (golden vectors in tests/golden/ for
+ * float and double weights, and a direct cross-check where oracle/_ref is
+ * built), Dijkstra bit-equality, brute force over all simple paths on tiny
+ * graphs (float, double and unsigned sums), closed forms (chains, stars,
+ * disconnected parts), edge-permutation invariance; the capped loop equals
+ * the full loop at a cap >= its pass count and bounds the fixpoint below it.
  *
  * Build: see oracle/Makefile (gcc -O3 -fopenmp, never -ffast-math: fp32 adds
  * must be single IEEE RNE operations for bit-exact parity).

@@ -84,6 +84,21 @@ int ref_cpu_bellman_ford_f32(const int32_t *src, const int32_t *dst, const float
     return 0;
 }
 
+// The same for double weights (the reference's other instantiated weight
+// type, bellman_ford.cu:150-156).
+int ref_cpu_bellman_ford_f64(const int32_t *src, const int32_t *dst, const double *w,
+                             int32_t V, int64_t E, int32_t source, int32_t threads,
+                             double *dist)
+{
+    try {
+        Graph<int, double> g(0, 0, EDGES_LIST, true);
+        fill_graph(g, src, dst, w, V, E);
+        SingleSourceShortestPaths<int, double> op(threads);
+        op.cpu_bellman_ford(g, dist, source);
+    } catch (const char *) { return 1; }
+    return 0;
+}
+
 // The same for unsigned int weights: the reference loop as written, including
 // its unguarded 0xFFFFFFFF + w (sssp.hpp:21,56). Used only to document that
 // defect next to the guarded oracle.

@@ -106,6 +106,35 @@ int sssp_device_info(int32_t device, char *name, int32_t name_len, int32_t *cu_c
     });
 }
 
+int sssp_device_props(int32_t device, char *name, int32_t name_len, int64_t *free_bytes,
+                      int64_t *total_bytes, int32_t *max_threads_per_block, int32_t *max_grid)
+{
+    return guard([&] {
+        int n = 0, prev = 0;
+        SSSP_HIP(hipGetDeviceCount(&n));
+        SSSP_REQUIRE(device >= 0 && device < n, "device index out of range");
+        SSSP_HIP(hipGetDevice(&prev));
+        struct Restore {
+            int d;
+            ~Restore() { (void)hipSetDevice(d); }
+        } restore{prev};
+        SSSP_HIP(hipSetDevice(device));
+        hipDeviceProp_t p;
+        SSSP_HIP(hipGetDeviceProperties(&p, device));
+        if (name && name_len > 0) {
+            std::strncpy(name, p.name, (size_t)name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        size_t fr = 0, tot = 0;
+        if (free_bytes || total_bytes) SSSP_HIP(hipMemGetInfo(&fr, &tot));
+        if (free_bytes) *free_bytes = (int64_t)fr;
+        if (total_bytes) *total_bytes = (int64_t)tot;
+        if (max_threads_per_block) *max_threads_per_block = p.maxThreadsPerBlock;
+        if (max_grid)
+            for (int i = 0; i < 3; i++) max_grid[i] = p.maxGridSize[i];
+    });
+}
+
 }  // extern "C"
 
 namespace sssp {
@@ -449,6 +478,7 @@ int sssp_bfs_edges(const int32_t *d_src, const int32_t *d_dst, int32_t vertices,
         SSSP_REQUIRE(d_visited && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst)),
                      "bad arguments");
         SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        validate_edges(d_src, d_dst, nullptr, -1, edges, vertices, nullptr);
         ref_bfs(d_src, d_dst, vertices, edges, d_visited, source, passes);
     });
 }
@@ -513,7 +543,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
             g->advance_blocks = (int)value;
         } else if (n == "verify") g->verify = value != 0;
         else if (n == "max_rounds") {
-            SSSP_REQUIRE(value >= 1, "max_rounds must be >= 1");
+            // round ticks of a solve (and twice that for a timing collection)
+            // must stay below 2^30
+            SSSP_REQUIRE(value >= 1 && value <= (double)(1 << 28), "max_rounds must be in [1, 2^28]");
             g->max_rounds = (int)value;
         } else if (n == "push_edge_frac") g->push_edge_frac = value;
         else if (n == "pull_frac") g->pull_frac = value;
@@ -523,6 +555,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "collect_kind") {
             SSSP_REQUIRE(value == 0 || value == 1, "collect_kind must be 0 (SSSP) or 1 (BFS)");
             g->collect_kind = (int)value;
+        } else if (n == "collect_budget") {
+            SSSP_REQUIRE(value >= 1 && value <= 9.0e15, "collect_budget must be >= 1");
+            g->collect_budget = (int64_t)value;
         } else if (n == "rollout_policy") {
             SSSP_REQUIRE(value == 0 || value == 1, "rollout_policy must be 0 (rule) or 1 (network)");
             g->rollout_policy = (int)value;
@@ -567,6 +602,8 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "dense_kernel") *value = g->dense_kernel;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;
+        else if (n == "collect_budget") *value = (double)g->collect_budget;
+        else if (n == "collect_rounds") *value = (double)g->collect_rounds;
         else if (n == "trace_events") *value = g->trace_events;
         else if (n == "cu_count") *value = g->cu_count;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};
@@ -607,6 +644,7 @@ int sssp_bellman_ford_f32(const int32_t *d_src, const int32_t *d_dst, const floa
         SSSP_REQUIRE(d_dist && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst && d_w)),
                      "bad arguments");
         SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        validate_edges(d_src, d_dst, d_w, SSSP_W_F32, edges, vertices, nullptr);
         ref_bf<WF32, uint32_t>(d_src, d_dst, reinterpret_cast<const uint32_t *>(d_w), vertices, edges,
                                reinterpret_cast<uint32_t *>(d_dist), passes);
     });
@@ -620,6 +658,7 @@ int sssp_bellman_ford_u32(const int32_t *d_src, const int32_t *d_dst, const uint
         SSSP_REQUIRE(d_dist && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst && d_w)),
                      "bad arguments");
         SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        validate_edges(d_src, d_dst, d_w, SSSP_W_U32, edges, vertices, nullptr);
         ref_bf<WU32, uint32_t>(d_src, d_dst, d_w, vertices, edges, d_dist, passes);
     });
 }
@@ -632,6 +671,7 @@ int sssp_bellman_ford_f64(const int32_t *d_src, const int32_t *d_dst, const doub
         SSSP_REQUIRE(d_dist && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst && d_w)),
                      "bad arguments");
         SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        validate_edges(d_src, d_dst, d_w, SSSP_W_F64, edges, vertices, nullptr);
         ref_bf<WF64, unsigned long long>(d_src, d_dst,
                                          reinterpret_cast<const unsigned long long *>(d_w), vertices,
                                          edges, reinterpret_cast<unsigned long long *>(d_dist),

@@ -1753,6 +1753,11 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
 // round trips included, as in a real solve). The solve then advances by the best c. The
 // labels are thus one step of policy improvement over the rule: a round that
 // costs more now but leaves less work scores by the total.
+// The cost is quadratic in the solve's round count R (each of R rounds runs
+// 12 choices x 2 timings of up to R remaining rounds), so a far-pile solve of
+// hundreds of rounds runs hundreds of thousands of them: the host counts
+// every round it launches and gives up with SSSP_ERR_NOT_CONVERGED past
+// g->collect_budget (between rounds, so every kernel has ended normally).
 constexpr int kRolloutReps = 2;
 
 template <class WT>
@@ -1808,6 +1813,17 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
     bool verified = false;
     int64_t relaxed = 0;
     sssp_stats st{};
+    g->collect_rounds = 0;
+    auto charge = [&] {
+        if (++g->collect_rounds > g->collect_budget) {
+            SSSP_HIP(hipStreamSynchronize(s));
+            g->tick = tick0 + 2 * g->max_rounds + 2;
+            g->trace_events = trace_was;
+            throw Error{SSSP_ERR_NOT_CONVERGED,
+                        "collect: round budget exhausted (" + std::to_string(g->collect_budget) +
+                            " rounds; raise collect_budget)"};
+        }
+    };
     auto copy = [&](void *dst, const void *src, size_t bytes) {
         SSSP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
     };
@@ -1850,6 +1866,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
                         bool ver = verified;  // the rollout continues this solve
                         int64_t rel = 0;
                         sssp_stats tmp{};
+                        charge();
                         if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &tmp, ver, rel, mode,
                                             c)) {
                             over = true;  // nothing pending at round r: the solve is complete
@@ -1868,6 +1885,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
                                 throw Error{SSSP_ERR_NOT_CONVERGED,
                                             "collect: rollout did not converge"};
                             }
+                            charge();
                             if (!step_round<WT>(g, r2, cont, tick0, &tmp, ver, rel))
                                 break;
                         }
@@ -1890,6 +1908,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
         }
         restore();
         if (over) break;
+        charge();
         if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &st, verified, relaxed, bm, bc))
             break;
         if (!verify_round && row < cap) {

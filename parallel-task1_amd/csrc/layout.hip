@@ -88,8 +88,10 @@ __global__ __launch_bounds__(kBlock) void k_validate(const int32_t *src, const i
          e += (int64_t)gridDim.x * kBlock) {
         int32_t s = src[e], d = dst[e];
         if (s < 0 || s >= V || d < 0 || d >= V) nb_id++;
-        W x = w[e];
-        if (!(x >= (W)0)) nb_w++;  // negative or NaN
+        if (w) {
+            const W x = w[e];
+            if (!(x >= (W)0)) nb_w++;  // negative or NaN
+        }
     }
     nb_id = wave_sum64(nb_id);
     nb_w = wave_sum64(nb_w);
@@ -535,6 +537,40 @@ void build_multi_layout(sssp_graph *g)
     g->n_mtile = (int32_t)tiles.size();
 }
 
+// Edge-list check on the device: every id in [0, V) and (wtype >= 0) every
+// weight >= 0 and not NaN (unsigned weights always are); throws SSSP_ERR_ARG.
+// The relaxations rely on it: distances are compared as unsigned bit patterns,
+// which orders non-negative IEEE values only.
+void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int wtype, int64_t E,
+                    int32_t V, hipStream_t s)
+{
+    if (E <= 0) return;
+    int64_t *bad = dmalloc<int64_t>(2);
+    struct Free {
+        void *p;
+        ~Free() { dfree(p); }
+    } fr{bad};
+    SSSP_HIP(hipMemsetAsync(bad, 0, 16, s));
+    if (wtype == SSSP_W_F32)
+        k_validate<float><<<grid_for(E), kBlock, 0, s>>>(src, dst, static_cast<const float *>(w), E,
+                                                         V, bad);
+    else if (wtype == SSSP_W_F64)
+        k_validate<double><<<grid_for(E), kBlock, 0, s>>>(src, dst, static_cast<const double *>(w),
+                                                          E, V, bad);
+    else if (wtype == SSSP_W_U32)
+        k_validate<uint32_t><<<grid_for(E), kBlock, 0, s>>>(src, dst,
+                                                            static_cast<const uint32_t *>(w), E, V,
+                                                            bad);
+    else
+        k_validate<uint32_t><<<grid_for(E), kBlock, 0, s>>>(src, dst, nullptr, E, V, bad);
+    SSSP_LAUNCH_CHECK("k_validate");
+    int64_t hb[2] = {0, 0};
+    SSSP_HIP(hipMemcpyAsync(hb, bad, 16, hipMemcpyDeviceToHost, s));
+    SSSP_HIP(hipStreamSynchronize(s));
+    if (hb[0]) throw Error{SSSP_ERR_ARG, std::to_string(hb[0]) + " edges have a vertex id outside [0, V)"};
+    if (hb[1]) throw Error{SSSP_ERR_ARG, std::to_string(hb[1]) + " edges have a negative or NaN weight"};
+}
+
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
                    bool build_csc)
 {
@@ -542,25 +578,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
     const int64_t E = g->E;
     const int32_t V = g->V;
 
-    // validation
-    {
-        int64_t *bad = dmalloc<int64_t>(2);
-        SSSP_HIP(hipMemsetAsync(bad, 0, 16, s));
-        if (E > 0) {
-            if (g->wtype == SSSP_W_F32)
-                k_validate<float><<<grid_for(E), kBlock, 0, s>>>(
-                    d_src, d_dst, reinterpret_cast<const float *>(d_w), E, V, bad);
-            else
-                k_validate<uint32_t><<<grid_for(E), kBlock, 0, s>>>(d_src, d_dst, d_w, E, V, bad);
-            SSSP_LAUNCH_CHECK("k_validate");
-        }
-        int64_t hb[2] = {0, 0};
-        SSSP_HIP(hipMemcpyAsync(hb, bad, 16, hipMemcpyDeviceToHost, s));
-        SSSP_HIP(hipStreamSynchronize(s));
-        dfree(bad);
-        if (hb[0]) throw Error{SSSP_ERR_ARG, std::to_string(hb[0]) + " edges have a vertex id outside [0, V)"};
-        if (hb[1]) throw Error{SSSP_ERR_ARG, std::to_string(hb[1]) + " edges have a negative or NaN weight"};
-    }
+    validate_edges(d_src, d_dst, d_w, g->wtype, E, V, s);
 
     // scratch for the sorts
     const int64_t rs_tiles = (E + kRsTile - 1) / kRsTile;

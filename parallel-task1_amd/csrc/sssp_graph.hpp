@@ -196,6 +196,8 @@ struct sssp_graph {
     bool last_round_dense = false;  // the last round ran dense (traces)
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds
+    int64_t collect_budget = (int64_t)1 << 22;  // rounds one timing collection may run in total
+    int64_t collect_rounds = 0;    // rounds the last timing collection ran
     int trace_events = 0;
 
     // last solve
@@ -263,6 +265,10 @@ void rmat_device(int scale, int ef, int a, int b, int c, int d, uint32_t base_se
                  int32_t *src, int32_t *dst, float *w, hipStream_t s);
 
 // layout.hip
+// ids in [0, V), and for wtype SSSP_W_F32/F64/U32 weights >= 0 and not NaN
+// (any other wtype: ids only); throws SSSP_ERR_ARG
+void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int wtype, int64_t E,
+                    int32_t V, hipStream_t s);
 void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s);
 // packed 12-byte binary edge-list records (device) -> src, dst, w (device)
 void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, uint32_t *w,

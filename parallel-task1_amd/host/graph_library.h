@@ -11,16 +11,25 @@
 //   SingleSourceShortestPaths<V,W>::gpu_bellman_ford  (algorithm/sssp/sssp.h:23-41)
 //   init_distances_wrapper<W> / bellman_ford_wrapper<W>
 //                           (algorithm/sssp/gpu/bellman_ford.cuh:13-18)
+//   SingleSourceShortestPaths<V,W>::cpu_bellman_ford  (sssp.hpp:3-71)
 //   BFS<V,W>::gpu_bfs and bfs_wrapper (algorithm/bfs/bfs.h:14-45,
 //                           algorithm/bfs/gpu/bfs.cuh:12)
+//   NodeData / init_library (include/node_data/node_data.h:26-92)
 // Everything device-side goes through the C ABI of libsssp_mi355x.so
 // (include/sssp_mi355x.h). Failures throw `const char*`, as the reference's
 // SAFE_CALL does (include/common/cuda_error_hadling.h:4-24).
+// include/graph_library.h next to this file is the umbrella the reference's
+// drivers include by that path (it adds the reference umbrella's standard
+// headers and `using namespace std`).
 #pragma once
+
+#include <omp.h>
+#include <sys/sysinfo.h>
 
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <iostream>
 #include <limits>
 #include <string>
 #include <vector>
@@ -35,6 +44,14 @@ inline void check(int rc)
         msg = std::string("error in HIP library call, aborting...: ") + sssp_last_error();
         throw msg.c_str();
     }
+}
+// d + w as the relaxation computes it: one IEEE add for floating types, an
+// add guarded at `inf` for integer types.
+template <typename W>
+inline W relax_sum(W d, W w, W inf)
+{
+    if (std::numeric_limits<W>::is_integer) return (W)(d > inf - w ? inf : d + w);
+    return d + w;
 }
 template <typename W> struct WeightType;
 template <> struct WeightType<float> { static const int32_t code = SSSP_W_F32; };
@@ -83,6 +100,7 @@ class Graph {
 
     void resize(int vertices_count, long long edges_count, bool empty = true)
     {
+        version_++;
         if (type_ == ADJACENCY_LIST) type_ = EDGES_LIST;
         values_.assign((size_t)vertices_count, _TVertexValue());
         ids_.clear();
@@ -112,6 +130,7 @@ class Graph {
     void add_edge(int src, int dst, _TEdgeWeight weight)
     {
         if (type_ != EDGES_LIST) throw "add_edge not implemented yet";
+        version_++;
         if (!directed_ && src > dst) std::swap(src, dst);  // edges_list.hpp:240-244
         if ((size_t)ne_ >= src_.size()) {
             size_t n = std::max<size_t>(1, src_.size() * 2);
@@ -132,8 +151,10 @@ class Graph {
         return Edge<_TEdgeWeight>(src_[(size_t)pos], dst_[(size_t)pos], w_[(size_t)pos]);
     }
 
+    // The caller may write through the returned pointers: counts as a change.
     GraphContainerData<_TVertexValue, _TEdgeWeight> get_graph_data()
     {
+        version_++;
         GraphContainerData<_TVertexValue, _TEdgeWeight> d;
         if (type_ == EDGES_LIST) {
             d.edges_src_ids = src_.data();
@@ -151,9 +172,19 @@ class Graph {
         return d;
     }
 
+    // Read-only view of the edge list (EDGES_LIST only), for the library's
+    // own use: unlike get_graph_data it does not count as a change.
+    const int *edge_src_view() const { return src_.data(); }
+    const int *edge_dst_view() const { return dst_.data(); }
+    const _TEdgeWeight *edge_weight_view() const { return w_.data(); }
+    // Bumped by every call that can change the edges (resize, add_edge,
+    // conversions, get_graph_data).
+    unsigned long long modification_count() const { return version_; }
+
     void convert_to_edges_list()
     {
         if (type_ == EDGES_LIST) return;
+        version_++;
         // CSR rows in vertex order back to an edge list
         for (int v = 0; v < nv_; v++)
             for (long long e = ptrs_[(size_t)v]; e < ptrs_[(size_t)v + 1]; e++) src_[(size_t)e] = v;
@@ -164,6 +195,7 @@ class Graph {
     void convert_to_compressed_adjacency_list()
     {
         if (type_ == COMPRESSED_ADJACENCY_LIST) return;
+        version_++;
         // stable counting sort of the edge list by source
         std::vector<long long> cnt((size_t)nv_ + 1, 0);
         for (long long e = 0; e < ne_; e++) cnt[(size_t)src_[(size_t)e] + 1]++;
@@ -201,6 +233,7 @@ class Graph {
     std::vector<int> src_, dst_;
     std::vector<_TEdgeWeight> w_;
     std::vector<long long> ptrs_;
+    unsigned long long version_ = 0;
 };
 
 template <typename _TVertexValue, typename _TEdgeWeight>
@@ -309,9 +342,38 @@ inline void bellman_ford_wrapper<unsigned int>(int *s, int *t, unsigned int *w, 
 }
 
 namespace sssp_host {
-// The device graph of the last Graph passed in, rebuilt only when its data
-// pointers or sizes change (the reference uploaded on every call,
-// sssp.hpp:245-253 and bfs.hpp:230-238).
+// Order-independent fingerprint of an edge list: the sum over edges of a
+// 64-bit mix of (src, dst, weight bits), plus the sizes. Two lists with the
+// same multiset of edges (hence the same distances) agree; an edited id or
+// weight changes it except with probability ~2^-64.
+template <typename W>
+unsigned long long edge_fingerprint(const int *src, const int *dst, const W *w, int V, long long E)
+{
+    auto mix = [](unsigned long long x) {
+        x ^= x >> 33;
+        x *= 0xff51afd7ed558ccdull;
+        x ^= x >> 33;
+        x *= 0xc4ceb9fe1a85ec53ull;
+        x ^= x >> 33;
+        return x;
+    };
+    unsigned long long sum = 0;
+#pragma omp parallel for reduction(+ : sum) schedule(static)
+    for (long long e = 0; e < E; e++) {
+        unsigned long long wb = 0;
+        std::memcpy(&wb, &w[e], sizeof(W));
+        const unsigned long long k =
+            ((unsigned long long)(unsigned)src[e] << 32) | (unsigned long long)(unsigned)dst[e];
+        sum += mix(k ^ mix(wb + 0x9e3779b97f4a7c15ull));
+    }
+    return mix(sum ^ ((unsigned long long)(unsigned)V << 40) ^ (unsigned long long)E);
+}
+
+// The device graph of the last Graph passed in. It is reused only while the
+// Graph reports no change (modification_count) and its edges still have the
+// same fingerprint -- the latter catches writes through pointers taken from
+// get_graph_data before an earlier solve; any difference re-uploads, as the
+// reference re-uploaded on every call (sssp.hpp:245-253, bfs.hpp:230-238).
 template <typename _TVertexValue, typename _TEdgeWeight>
 class DeviceGraphCache {
   public:
@@ -326,20 +388,22 @@ class DeviceGraphCache {
     sssp_graph *get(Graph<_TVertexValue, _TEdgeWeight> &graph, int device, sssp_mlp *mlp)
     {
         graph.convert_to_edges_list();
-        GraphContainerData<_TVertexValue, _TEdgeWeight> d = graph.get_graph_data();
+        const int *src = graph.edge_src_view();
+        const int *dst = graph.edge_dst_view();
+        const _TEdgeWeight *w = graph.edge_weight_view();
         const int V = graph.get_vertices_count();
         const long long E = graph.get_edges_count();
-        if (!g_ || key_src_ != d.edges_src_ids || key_dst_ != d.edges_dst_ids ||
-            key_w_ != (const void *)d.edges_weights || key_v_ != V || key_e_ != E) {
+        const unsigned long long fp = edge_fingerprint(src, dst, w, V, E);
+        if (!g_ || graph_ != &graph || version_ != graph.modification_count() || fp_ != fp ||
+            device_ != device) {
             if (g_) sssp_graph_destroy(g_);
             g_ = NULL;
-            check(sssp_graph_create(WeightType<_TEdgeWeight>::code, d.edges_src_ids,
-                                    d.edges_dst_ids, d.edges_weights, V, E, device, 0, &g_));
-            key_src_ = d.edges_src_ids;
-            key_dst_ = d.edges_dst_ids;
-            key_w_ = d.edges_weights;
-            key_v_ = V;
-            key_e_ = E;
+            check(sssp_graph_create(WeightType<_TEdgeWeight>::code, src, dst, w, V, E, device, 0,
+                                    &g_));
+            graph_ = &graph;
+            version_ = graph.modification_count();
+            fp_ = fp;
+            device_ = device;
         }
         if (mlp) check(sssp_graph_set_mlp(g_, mlp));
         return g_;
@@ -348,11 +412,10 @@ class DeviceGraphCache {
 
   private:
     sssp_graph *g_ = NULL;
-    const int *key_src_ = NULL;
-    const int *key_dst_ = NULL;
-    const void *key_w_ = NULL;
-    int key_v_ = 0;
-    long long key_e_ = 0;
+    const void *graph_ = NULL;
+    unsigned long long version_ = 0;
+    unsigned long long fp_ = 0;
+    int device_ = -1;
 };
 }  // namespace sssp_host
 
@@ -376,6 +439,64 @@ class SingleSourceShortestPaths {
 
     void set_omp_threads(int n) { omp_threads_ = n; }
     void set_schedule(int schedule) { schedule_ = schedule; }
+
+    // cpu_bellman_ford (sssp.hpp:3-71): the reference's host path, kept so that
+    // drivers calling it (sample/sssp.cpp:120) run unchanged. Distances start
+    // at the type's max (0 at the source); passes over the edge list in its
+    // order, `omp for schedule(guided)` over omp_threads threads, relax in
+    // place while d[dst] > d[src] + w, until a pass changes nothing. The
+    // distance words are read and written with OpenMP atomics (a lost race
+    // only costs another pass, as in the reference; no torn values). For
+    // unsigned weights the add is guarded at 0xFFFFFFFF (the reference wraps,
+    // sssp.hpp:21,56), as on the device.
+    void cpu_bellman_ford(Graph<_TVertexValue, _TEdgeWeight> &graph, _TEdgeWeight *distances,
+                          int source_vertex)
+    {
+        graph.convert_to_edges_list();
+        const int *src = graph.edge_src_view();
+        const int *dst = graph.edge_dst_view();
+        const _TEdgeWeight *w = graph.edge_weight_view();
+        const int V = graph.get_vertices_count();
+        const long long E = graph.get_edges_count();
+        if (source_vertex < 0 || source_vertex >= V) throw "source vertex out of range";
+        const _TEdgeWeight inf = std::numeric_limits<_TEdgeWeight>::max();
+        int changes = 1;
+        int threads = omp_threads_ > 0 ? omp_threads_ : 1;
+#pragma omp parallel num_threads(threads)
+        {
+#pragma omp for schedule(static)
+            for (int i = 0; i < V; i++) distances[i] = inf;
+#pragma omp single
+            distances[source_vertex] = 0;
+            for (;;) {
+                int go;
+#pragma omp atomic read
+                go = changes;
+                if (!go) break;
+#pragma omp barrier
+#pragma omp single
+                changes = 0;  // the single's barrier orders it before the pass
+#pragma omp for schedule(guided)
+                for (long long e = 0; e < E; e++) {
+                    _TEdgeWeight ds, dd;
+#pragma omp atomic read
+                    ds = distances[src[e]];
+                    if (!std::numeric_limits<_TEdgeWeight>::is_integer || ds != inf) {
+                        const _TEdgeWeight cand = sssp_host::relax_sum(ds, w[e], inf);
+#pragma omp atomic read
+                        dd = distances[dst[e]];
+                        if (dd > cand) {
+#pragma omp atomic write
+                            distances[dst[e]] = cand;
+#pragma omp atomic write
+                            changes = 1;
+                        }
+                    }
+                }
+                // implicit barrier of the omp for: every thread sees the flag
+            }
+        }
+    }
 
     // Loads scheduler weights (SSSP_MLP_NPARAM floats) for SSSP_SCHED_LEARNED.
     void set_scheduler_weights(const float *params)
@@ -458,3 +579,109 @@ class BFS {
     sssp_mlp *mlp_ = NULL;
     sssp_host::DeviceGraphCache<_TVertexValue, _TEdgeWeight> cache_;
 };
+
+// NodeData / init_library (include/node_data/node_data.h:26-92,
+// node_data.cpp:15-147): host RAM from sysinfo, and per device its name, free
+// and total global memory and launch limits, through the C ABI. Unlike the
+// reference, init_library leaves the current device as it was (the library
+// takes the device explicitly everywhere).
+struct GridSize {
+    int x, y, z;
+};
+
+class NodeData {
+  public:
+    NodeData() {}
+    void print_node_properties()
+    {
+        std::cout << "--------------------------- Init node result: -------------------------------"
+                  << std::endl
+                  << std::endl;
+        std::cout << "Total host memory size: " << host_total_memory_ / (1024ull * 1024 * 1024)
+                  << " GB" << std::endl;
+        std::cout << "Free host memory size: " << host_free_memory_ / (1024ull * 1024 * 1024)
+                  << " GB" << std::endl
+                  << std::endl;
+        std::cout << "There are " << devices_count_ << " devices on current node:" << std::endl;
+        for (int i = 0; i < devices_count_; i++) {
+            std::cout << names_[(size_t)i] << std::endl;
+            std::cout << (double)device_free_memory_[(size_t)i] / (1024.0 * 1024.0)
+                      << " Mb of global memory free " << std::endl;
+            std::cout << (double)device_total_memory_[(size_t)i] / (1024.0 * 1024.0)
+                      << " Mb of global memory total " << std::endl;
+        }
+        std::cout << std::endl
+                  << "-----------------------------------------------------------------------------"
+                  << std::endl
+                  << std::endl;
+    }
+    size_t get_host_total_memory() { return host_total_memory_; }
+    size_t get_host_free_memory() { return host_free_memory_; }
+    int get_devices_count() { return devices_count_; }
+    size_t get_device_free_memory(int d) { return device_free_memory_.at((size_t)d); }
+    size_t get_device_total_memory(int d) { return device_total_memory_.at((size_t)d); }
+    int get_max_threads_per_block(int d) { return max_threads_.at((size_t)d); }
+    GridSize get_max_grid_size(int d) { return max_grid_.at((size_t)d); }
+    void update_host_memory_data() { collect_host_properties(); }
+    void update_device_memory_data()
+    {
+        for (int i = 0; i < devices_count_; i++) {
+            int64_t fr = 0, tot = 0;
+            sssp_host::check(sssp_device_props(i, NULL, 0, &fr, &tot, NULL, NULL));
+            device_free_memory_[(size_t)i] = (size_t)fr;
+            device_total_memory_[(size_t)i] = (size_t)tot;
+        }
+    }
+    void set_test_free_memory(int d, size_t bytes) { device_free_memory_.at((size_t)d) = bytes; }
+
+    friend void init_library(NodeData &node_data);
+
+  private:
+    void collect_host_properties()
+    {
+        struct sysinfo info;
+        if (sysinfo(&info) == 0) {
+            host_total_memory_ = (size_t)info.totalram * info.mem_unit;
+            host_free_memory_ = (size_t)info.freeram * info.mem_unit;
+        }
+    }
+    void collect_devices_properties()
+    {
+        int32_t n = 0;
+        names_.clear();
+        device_free_memory_.clear();
+        device_total_memory_.clear();
+        max_threads_.clear();
+        max_grid_.clear();
+        if (sssp_device_count(&n) != SSSP_OK || n < 1) {
+            std::cout << "No GPU devices installed in the current system." << std::endl;
+            devices_count_ = 0;
+            return;
+        }
+        devices_count_ = n;
+        for (int i = 0; i < n; i++) {
+            char name[256];
+            int64_t fr = 0, tot = 0;
+            int32_t mt = 0, mg[3] = {0, 0, 0};
+            sssp_host::check(sssp_device_props(i, name, (int32_t)sizeof(name), &fr, &tot, &mt, mg));
+            names_.push_back(name);
+            device_free_memory_.push_back((size_t)fr);
+            device_total_memory_.push_back((size_t)tot);
+            max_threads_.push_back(mt);
+            max_grid_.push_back(GridSize{mg[0], mg[1], mg[2]});
+        }
+    }
+
+    size_t host_total_memory_ = 0, host_free_memory_ = 0;
+    int devices_count_ = 0;
+    std::vector<std::string> names_;
+    std::vector<size_t> device_free_memory_, device_total_memory_;
+    std::vector<int> max_threads_;
+    std::vector<GridSize> max_grid_;
+};
+
+inline void init_library(NodeData &node_data)
+{
+    node_data.collect_host_properties();
+    node_data.collect_devices_properties();
+}

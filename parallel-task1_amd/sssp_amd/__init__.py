@@ -65,6 +65,7 @@ _SIGS = {
     "sssp_version": (I32, []),
     "sssp_device_count": (I32, [PI32]),
     "sssp_device_info": (I32, [I32, C.c_char_p, I32, PI32, PI64, PI32]),
+    "sssp_device_props": (I32, [I32, C.c_char_p, I32, PI64, PI64, PI32, PI32]),
     "sssp_graph_create": (I32, [I32, P, P, P, I32, I64, I32, U32, C.POINTER(P)]),
     "sssp_graph_create_from_bin": (I32, [C.c_char_p, I32, I32, I32, U32, C.POINTER(P)]),
     "sssp_graph_create_rmat": (I32, [I32, I32, I32, I32, I32, I32, U32, I32, I32, U32, P, P, P,
@@ -133,9 +134,14 @@ def exported_symbols():
     return list(_SIGS)
 
 
+def last_error() -> str:
+    """Message of the last failed call on this thread (sssp_last_error)."""
+    return lib().sssp_last_error().decode(errors="replace")
+
+
 def _check(rc):
     if rc != OK:
-        raise SSSPError(rc, lib().sssp_last_error().decode(errors="replace"))
+        raise SSSPError(rc, last_error())
 
 
 def _ptr(a):
@@ -535,6 +541,29 @@ class EdgesListGraph:
         return len(self.src_ids)
 
 
+def edge_fingerprint(src, dst, w, chunk=1 << 22) -> int:
+    """Order-independent 64-bit fingerprint of an edge list: the sum over
+    edges of a mix of (src, dst, weight bits). Lists with the same multiset of
+    edges agree; an edited id or weight changes it (up to ~2^-64)."""
+    def mix(x):
+        x = x ^ (x >> np.uint64(33))
+        x = x * np.uint64(0xff51afd7ed558ccd)
+        x = x ^ (x >> np.uint64(33))
+        x = x * np.uint64(0xc4ceb9fe1a85ec53)
+        return x ^ (x >> np.uint64(33))
+
+    wv = np.ascontiguousarray(w)
+    wv = wv.view(np.uint32 if wv.itemsize == 4 else np.uint64).astype(np.uint64)
+    total = np.uint64(0)
+    with np.errstate(over="ignore"):
+        for i in range(0, len(src), chunk):
+            k = (np.asarray(src[i:i + chunk]).astype(np.uint32).astype(np.uint64) << np.uint64(32)) \
+                | np.asarray(dst[i:i + chunk]).astype(np.uint32).astype(np.uint64)
+            h = mix(k ^ mix(wv[i:i + chunk] + np.uint64(0x9e3779b97f4a7c15)))
+            total = total + np.add.reduce(h, dtype=np.uint64)
+    return int(total)
+
+
 class GraphGenerationAPI:
     @staticmethod
     def R_MAT_parallel(vertices_count, average_connections, a, b, c, d, omp_threads,
@@ -605,8 +634,12 @@ class SingleSourceShortestPaths:
         self.omp_threads = n
 
     def _device_graph(self, graph: EdgesListGraph):
+        # reused only while the arrays and their contents are unchanged: an
+        # edit in place changes the fingerprint and the graph is uploaded again
+        # (the reference uploaded on every call, sssp.hpp:245-253)
         key = (id(graph.src_ids), id(graph.dst_ids), id(graph.weights), graph.vertices_count,
-               graph.get_edges_count())
+               graph.get_edges_count(),
+               edge_fingerprint(graph.src_ids, graph.dst_ids, graph.weights))
         g = self._cache.get(key)
         if g is None:
             self._cache.clear()

@@ -40,6 +40,8 @@ class Oracle:
         for n in ("orc_certify_f32", "orc_certify_u32"):
             getattr(L, n).argtypes = [p, p, p, i32, i64, i32, p, p, p]
             getattr(L, n).restype = C.c_int
+        L.orc_bf_f32_capped.argtypes = [p, p, p, i32, i64, i32, i32, i32, p, p]
+        L.orc_bf_f32_capped.restype = C.c_int
         L.orc_max_threads.restype = C.c_int
         L.orc_wtime.restype = C.c_double
         L.orc_bfs.argtypes = [p, p, i32, i64, i32, p, p]
@@ -64,6 +66,15 @@ class Oracle:
         it = getattr(self.L, "orc_bf_" + kind)(_ptr(src), _ptr(dst), _ptr(w), V, len(src), source,
                                                threads, _ptr(d))
         return d, it
+
+    def bf_capped(self, src, dst, w, V, source, max_passes, threads=0):
+        """(distances, passes run, converged) of the pass-capped f32 loop."""
+        d = np.empty(V, np.float32)
+        conv = C.c_int32(-1)
+        it = self.L.orc_bf_f32_capped(_ptr(src), _ptr(dst), _ptr(np.ascontiguousarray(w)), V,
+                                      len(src), source, threads, max_passes, _ptr(d),
+                                      C.byref(conv))
+        return d, it, conv.value
 
     def dijkstra(self, src, dst, w, V, source):
         w = np.ascontiguousarray(w)
@@ -105,6 +116,7 @@ def ref_lib():
     L.ref_random_uniform_f32.argtypes = [i32, i32, C.c_uint32, p, p, p]
     L.ref_cpu_bellman_ford_f32.argtypes = [p, p, p, i32, i64, i32, i32, p]
     L.ref_cpu_bellman_ford_u32.argtypes = [p, p, p, i32, i64, i32, i32, p]
+    L.ref_cpu_bellman_ford_f64.argtypes = [p, p, p, i32, i64, i32, i32, p]
     L.ref_save_bin_f32.argtypes = [C.c_char_p, p, p, p, i32, i64]
     L.ref_load_bin_f32.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i64), p, p, p]
     L.ref_cpu_sequential_bfs.argtypes = [p, p, i32, i64, i32, p]
@@ -151,3 +163,38 @@ def small_graphs():
     # single vertex, no edges
     gs.append(("empty", 1, np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32)))
     return gs
+
+
+class HipBuffer:
+    """Device memory for the tests of the reference-shaped wrappers, which take
+    caller-owned device pointers (hipMalloc / hipMemcpy through ctypes)."""
+
+    _hip = None
+
+    def __init__(self, host: np.ndarray):
+        if HipBuffer._hip is None:
+            HipBuffer._hip = C.CDLL("libamdhip64.so")
+        self.dtype, self.n = host.dtype, host.size
+        self.nbytes = max(host.nbytes, 1)
+        self.ptr = C.c_void_p()
+        assert self._hip.hipMalloc(C.byref(self.ptr), C.c_size_t(self.nbytes)) == 0
+        if host.nbytes:
+            assert self._hip.hipMemcpy(self.ptr, _ptr(np.ascontiguousarray(host)),
+                                       C.c_size_t(host.nbytes), 1) == 0  # host to device
+
+    def get(self):
+        out = np.empty(self.n, self.dtype)
+        if out.nbytes:
+            assert self._hip.hipMemcpy(_ptr(out), self.ptr, C.c_size_t(out.nbytes), 2) == 0
+        return out
+
+    def free(self):
+        if self.ptr.value:
+            self._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()

@@ -1,6 +1,7 @@
 // Driver in the shape of the reference's sample/sssp.cpp / sample/solve.cpp,
 // compiled against parallel-task1_amd/host/graph_library.h:
-//   sssp_driver cpu <dir>        containers, generator and storage only
+//   sssp_driver cpu <dir>        containers, generator, storage, NodeData and
+//                                cpu_bellman_ford
 //   sssp_driver gpu <dir>        plus gpu_bellman_ford and the reference-shaped
 //                                device wrappers; writes distances to <dir>
 #include <cstdio>
@@ -20,7 +21,7 @@ static void write_f32(const std::string &path, const float *d, int n)
     std::fclose(f);
 }
 
-static void write_bytes(const std::string &path, const bool *d, int n)
+static void write_bytes(const std::string &path, const void *d, long n)
 {
     FILE *f = std::fopen(path.c_str(), "wb");
     if (!f) throw "cannot write output";
@@ -66,6 +67,26 @@ int main(int argc, char **argv)
         if (cd.vertices_to_edges_ptrs[V] != (long long)V * ef) throw "CSR total differs";
         csr.convert_to_edges_list();
         std::printf("containers, generator and storage ok\n");
+
+        // the reference driver's host path (sample/sssp.cpp:74-76, :111-120):
+        // node properties, then cpu_bellman_ford on all OpenMP threads
+        NodeData node_data;
+        init_library(node_data);
+        node_data.print_node_properties();
+        SingleSourceShortestPaths<int, float> cpu_op;
+        cpu_op.set_omp_threads(omp_get_max_threads());
+        std::vector<float> cpu_result((size_t)V);
+        cpu_op.cpu_bellman_ford(graph, cpu_result.data(), 5);
+        write_f32(dir + "/cpu_5.f32", cpu_result.data(), V);
+        // integer weights: the same loop with the guarded add
+        Graph<int, unsigned int> gu(0, 0, EDGES_LIST, true);
+        GraphGenerationAPI<int, unsigned int>::random_uniform(gu, 1000, 6, true, false, 17);
+        SingleSourceShortestPaths<int, unsigned int> cpu_u(4);
+        std::vector<unsigned int> du(1000);
+        cpu_u.cpu_bellman_ford(gu, du.data(), 0);
+        GraphStorageAPI<int, unsigned int>::save_to_edges_list_bin_file(gu, dir + "/er_u32.bin");
+        write_bytes(dir + "/cpu_u32_0.u32", du.data(), 4 * 1000);
+        std::printf("cpu path ok\n");
         if (!gpu) return 0;
 
         // the reference driver's loop, GPU path (sample/sssp.cpp:110-151)
@@ -80,6 +101,22 @@ int main(int argc, char **argv)
         }
         std::printf("avg perfomance: %.1f MTEPS\n", mteps / 3);
 
+        // the cached device graph follows edits of the Graph: a weight changed
+        // through fresh get_graph_data pointers, and one changed through
+        // pointers taken before the last solve
+        float *w_held = graph.get_graph_data().edges_weights;
+        {
+            GraphContainerData<int, float> e = graph.get_graph_data();
+            for (long long i = 0; i < graph.get_edges_count(); i++)
+                if (e.edges_src_ids[i] == 5) e.edges_weights[i] = 0.0f;  // every out-edge of 5
+        }
+        operation.gpu_bellman_ford(graph, result.data(), 5);
+        write_f32(dir + "/edit1_5.f32", result.data(), V);
+        for (long long i = 0; i < graph.get_edges_count(); i += 3) w_held[i] *= 0.5f;
+        operation.gpu_bellman_ford(graph, result.data(), 5);
+        write_f32(dir + "/edit2_5.f32", result.data(), V);
+        GraphStorageAPI<int, float>::save_to_edges_list_bin_file(graph, dir + "/edited.bin");
+
         // reference-shaped wrappers on device arrays owned by the caller
         GraphContainerData<int, float> d = graph.get_graph_data();
         const long long E = graph.get_edges_count();
@@ -92,7 +129,7 @@ int main(int argc, char **argv)
             hipMemcpy(dd, d.edges_dst_ids, E * 4, hipMemcpyHostToDevice) ||
             hipMemcpy(dw, d.edges_weights, E * 4, hipMemcpyHostToDevice))
             throw "hipMemcpy failed";
-        init_distances_wrapper<float>(dist, V, 5);
+        init_distances_wrapper<float>(dist, V, 5);  // on the edited graph
         bellman_ford_wrapper<float>(ds, dd, dw, V, E, 5, dist);
         if (hipMemcpy(result.data(), dist, V * 4, hipMemcpyDeviceToHost)) throw "hipMemcpy failed";
         write_f32(dir + "/wrap_5.f32", result.data(), V);

@@ -3,7 +3,8 @@
 Runs in the build container only: it loads oracle/_ref/libref_sssp.so, which
 `make -C oracle ref` compiles from the reference sources in /root/reference
 (GraphGenerationAPI::R_MAT_parallel and random_uniform,
-SingleSourceShortestPaths::cpu_bellman_ford, BFS::cpu_sequential_bfs,
+SingleSourceShortestPaths::cpu_bellman_ford for float, double and unsigned
+weights, BFS::cpu_sequential_bfs,
 GraphStorageAPI edge-list I/O).
 The outputs are data only (inputs and expected outputs).
 
@@ -40,6 +41,7 @@ def main():
     L.ref_random_uniform_f32.argtypes = [i32, i32, C.c_uint32, p, p, p]
     L.ref_cpu_bellman_ford_f32.argtypes = [p, p, p, i32, i64, i32, i32, p]
     L.ref_cpu_bellman_ford_u32.argtypes = [p, p, p, i32, i64, i32, i32, p]
+    L.ref_cpu_bellman_ford_f64.argtypes = [p, p, p, i32, i64, i32, i32, p]
     L.ref_save_bin_f32.argtypes = [C.c_char_p, p, p, p, i32, i64]
     L.ref_cpu_sequential_bfs.argtypes = [p, p, i32, i64, i32, p]
 
@@ -64,6 +66,23 @@ def main():
         dists = np.stack([bf(s, d, w, V, int(x)) for x in srcs])
         np.savez(os.path.join(HERE, f"rmat_v{V}_ef{ef}.npz"), src=s, dst=d, w=w, sources=srcs,
                  dist=dists)
+
+    # 1b. double weights (the reference's other instantiated type): the tiny
+    #     R-MAT topologies with full-precision uniform doubles, one zero weight
+    #     and one weight far below an ulp of the distances it is added to
+    for V, ef in [(256, 4), (1024, 8)]:
+        s, d, _ = rmat(V, ef)
+        w64 = np.random.default_rng(V).random(len(s))
+        w64[3] = 0.0
+        w64[5] = 1e-300
+        srcs = np.array([0, 1, int(s[len(s) // 3])], np.int32)
+        dists = []
+        for x in srcs:
+            out = np.empty(V, np.float64)
+            assert L.ref_cpu_bellman_ford_f64(P(s), P(d), P(w64), V, len(s), int(x), 1, P(out)) == 0
+            dists.append(out)
+        np.savez(os.path.join(HERE, f"rmat_v{V}_ef{ef}_f64.npz"), src=s, dst=d, w=w64,
+                 sources=srcs, dist=np.stack(dists))
 
     # 2. the CPU-runnable configuration: R-MAT scale 16, edge factor 16,
     #    one generator thread (seed 0), source 0 -- hashes plus samples

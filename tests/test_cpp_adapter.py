@@ -1,7 +1,9 @@
 """The C++ host mirror (parallel-task1_amd/host/graph_library.h) compiles with
 driver code in the shape of the reference's sample/sssp.cpp and runs it:
-containers, generator and storage on the CPU; gpu_bellman_ford, gpu_bfs and the
-reference-shaped device wrappers on the GPU, checked against the oracle."""
+containers, generator, storage, NodeData and cpu_bellman_ford on the CPU;
+gpu_bellman_ford (with edits of the cached graph), gpu_bfs and the
+reference-shaped device wrappers on the GPU, checked against the oracle. The
+reference's own sample/sssp.cpp compiles against it unchanged."""
 import os
 import subprocess
 
@@ -18,7 +20,7 @@ LIBDIR = os.path.join(ROOT, "parallel-task1_amd", "lib")
 @pytest.fixture(scope="module")
 def driver(tmp_path_factory):
     out = tmp_path_factory.mktemp("drv") / "sssp_driver"
-    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-fopenmp", "-D__HIP_PLATFORM_AMD__",
            "-I" + os.path.join(ROOT, "parallel-task1_amd", "host"), "-I/opt/rocm/include", SRC,
            "-L" + LIBDIR, "-lsssp_mi355x", "-L/opt/rocm/lib", "-lamdhip64",
            "-Wl,-rpath," + LIBDIR, "-o", str(out)]
@@ -26,7 +28,7 @@ def driver(tmp_path_factory):
     return str(out)
 
 
-def test_driver_compiles_and_runs_host_parts(driver, tmp_path):
+def test_driver_compiles_and_runs_host_parts(driver, oracle, tmp_path):
     r = subprocess.run([driver, "cpu", str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "containers, generator and storage ok" in r.stdout
@@ -34,6 +36,36 @@ def test_driver_compiles_and_runs_host_parts(driver, tmp_path):
     d, V, s, t, w = S.load_bin(tmp_path / "rmat12.bin")
     s2, t2, w2 = S.rmat(12, 16, seed=11, threads=64)
     assert V == 4096 and np.array_equal(s, s2) and np.array_equal(bits(w), bits(w2))
+    # NodeData printed the reference's report; cpu_bellman_ford (the mirror's
+    # own OpenMP loop, not the oracle) lands on the oracle's fixpoint
+    assert "Init node result" in r.stdout and "Total host memory size" in r.stdout
+    assert "cpu path ok" in r.stdout
+    got = np.fromfile(tmp_path / "cpu_5.f32", np.float32)
+    assert np.array_equal(bits(got), bits(oracle.bf(s, t, w, V, 5)[0]))
+    _, Vu, su, tu, wu = S.load_bin(tmp_path / "er_u32.bin", weight_dtype=np.uint32)
+    gu = np.fromfile(tmp_path / "cpu_u32_0.u32", np.uint32)
+    assert Vu == 1000 and np.array_equal(gu, oracle.dijkstra(su, tu, wu, Vu, 0))
+    assert (gu == 0xFFFFFFFF).any()  # unreached vertices keep the guarded maximum
+
+
+SAMPLE = "/root/reference/sample/sssp.cpp"
+
+
+@pytest.mark.skipif(not os.path.exists(SAMPLE), reason="reference tree not present")
+def test_reference_sample_sssp_compiles_against_the_mirror(tmp_path):
+    """The reference's own driver, read where it lies, compiles and links
+    unchanged against the mirror: its `#include "include/graph_library.h"`
+    resolves to parallel-task1_amd/host/include/graph_library.h (NodeData,
+    init_library, Graph, GraphStorageAPI, SingleSourceShortestPaths with
+    cpu_bellman_ford, using namespace std). It is not run: its main loop
+    solves for 55 minutes (sample/sssp.cpp:64, :117-137)."""
+    out = tmp_path / "sssp_sample"
+    cmd = ["g++", "-std=c++17", "-O1", "-fopenmp",
+           "-I" + os.path.join(ROOT, "parallel-task1_amd", "host"), SAMPLE, "-L" + LIBDIR,
+           "-lsssp_mi355x", "-Wl,-rpath," + LIBDIR, "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert out.exists()
 
 
 @pytest.mark.gpu
@@ -45,8 +77,25 @@ def test_driver_gpu_path_matches_oracle(gpu, oracle, driver, tmp_path):
     for src in (0, 5, 77):
         got = np.fromfile(tmp_path / f"dist_{src}.f32", np.float32)
         assert np.array_equal(bits(got), bits(oracle.dijkstra(s, t, w, V, src)))
+    # edits of the Graph reach the cached device graph: all out-edges of 5 set
+    # to 0 (through fresh get_graph_data pointers), then every third weight
+    # halved through pointers taken before the previous solve
+    w1 = w.copy()
+    w1[s == 5] = 0
+    got = np.fromfile(tmp_path / "edit1_5.f32", np.float32)
+    assert np.array_equal(bits(got), bits(oracle.dijkstra(s, t, w1, V, 5)))
+    assert not np.array_equal(bits(got), bits(oracle.dijkstra(s, t, w, V, 5)))
+    _, _, s2, t2, w2 = S.load_bin(tmp_path / "edited.bin")
+    w2x = w1.copy()
+    w2x[::3] *= np.float32(0.5)
+    assert np.array_equal(bits(w2), bits(w2x))
+    got = np.fromfile(tmp_path / "edit2_5.f32", np.float32)
+    ref2 = oracle.dijkstra(s2, t2, w2, V, 5)
+    assert np.array_equal(bits(got), bits(ref2))
+    assert not np.array_equal(bits(got), bits(oracle.dijkstra(s, t, w1, V, 5)))
     got = np.fromfile(tmp_path / "wrap_5.f32", np.float32)
-    assert np.array_equal(bits(got), bits(oracle.dijkstra(s, t, w, V, 5)))
+    assert np.array_equal(bits(got), bits(ref2))
+    s, t, w = s2, t2, w2
     # BFS<int,float>::gpu_bfs from 5 and bfs_wrapper from 77 against the oracle
     for name, src in (("bfs_5.u8", 5), ("bfsw_77.u8", 77)):
         got = np.fromfile(tmp_path / name, np.uint8)

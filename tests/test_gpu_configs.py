@@ -10,6 +10,10 @@
   C4b  8192 R-MAT graphs, scales uniform in {8..16} (numpy default_rng(5)),
        edge factor 8, graph k seeded 5 + k with one virtual thread; one source
        per graph: its smallest vertex with out-edges (vertex 0 when it has one)
+  C5   R-MAT scale 25, edge factor 16 (E = 537 M), base seed 6, 64 virtual
+       generator threads, generated on the device; the bench's 8 sources
+       (pick_sources(seed 6) over the vertices with out-edges); learned,
+       heuristic and plain schedules
 
 At these sizes the oracle (Dijkstra, oracle/oracle.c) runs on a sample; every
 result is also checked through properties that hold at any size: the GPU
@@ -213,3 +217,46 @@ def test_maximum_edge_count(gpu):
                         "plain": round(_gteps(g.E, sp["t_solve_ms"]), 2)}, "GTEPS")
     with pytest.raises(S.SSSPError):
         S.DeviceGraph.rmat(26, 32, seed=3, threads=64, csc=False)  # 2^31 edges: rejected
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_c5_rmat25_learned_heuristic_plain(gpu, oracle):
+    """The metric's own workload: for each of the bench's 8 sources the learned,
+    heuristic and plain schedules give the same bits and the GPU certificate
+    holds; the first source also equals the oracle's restatement of
+    cpu_bellman_ford (a full CPU solve on the host's OpenMP threads)."""
+    g, (src, dst, w) = S.DeviceGraph.rmat(25, 16, seed=6, threads=64, download_edges=True)
+    with g:
+        V, E = g.V, g.E
+        assert V == 1 << 25 and E == 536870912
+        m = _with_network(g)
+        deg = np.diff(g.layout(csc=False)["row_ptr"])
+        sources = S.pick_sources(np.nonzero(deg)[0].astype(np.int32), V, 6, 8)
+        t = {"learned": [], "heuristic": [], "plain": []}
+        first = None
+        for s in sources:
+            print(f"C5 source {int(s)}", flush=True)
+            ref = None
+            for sched in t:
+                d, st = g.solve(int(s), sched)
+                t[sched].append(st["t_solve_ms"])
+                if ref is None:
+                    ref = d
+                    ok, counts, _ = g.certify(int(s), parents=False)
+                    assert ok, (int(s), counts)
+                    assert counts[5] == st["reached"]
+                else:
+                    assert np.array_equal(bits(d), bits(ref)), (int(s), sched)
+            if first is None:
+                first = ref
+        dc, passes, sec = _cpu_bf(oracle, src, dst, w, V, int(sources[0]))
+        assert np.array_equal(bits(dc), bits(first))
+        gt = {k: round(_gteps(E * len(v), sum(v)), 2) for k, v in t.items()}
+        ratio = sum(t["heuristic"]) / sum(t["learned"])
+        print("C5", gt, "GTEPS; learned/heuristic", round(ratio, 3), "learned/plain",
+              round(sum(t["plain"]) / sum(t["learned"]), 3), "; CPU", round(E / sec / 1e9, 3),
+              "GTEPS (1 source,", passes, "passes)")
+        # the schedule must beat the plain passes by the target margin
+        assert sum(t["plain"]) / sum(t["learned"]) >= 1.5
+        del m

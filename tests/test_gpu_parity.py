@@ -370,8 +370,34 @@ def test_far_pile_after_failed_solve_and_collection(gpu, oracle):
         assert np.array_equal(bits(d), bits(ref))
         feats, times = g.collect_timings(3)
         assert len(feats) > 0 and np.all(np.isfinite(times[:, :8]))
+        print(f"collection: {len(feats)} rows, {int(g.get_param('collect_rounds'))} rounds run",
+              flush=True)
         d, _ = g.solve(3, "heuristic")
         assert np.array_equal(bits(d), bits(ref))
+        # a collection past its round budget gives up from the host between
+        # rounds (every kernel has ended); the graph stays usable
+        g.set_param("collect_budget", 40)
+        with pytest.raises(S.SSSPError) as e:
+            g.collect_timings(3)
+        assert e.value.code == S.ERR_NOT_CONVERGED and "budget" in str(e.value)
+        assert g.get_param("collect_rounds") == 41
+        d, _ = g.solve(3, "push")
+        assert np.array_equal(bits(d), bits(ref))
+        g.set_param("collect_budget", 1 << 22)
+
+
+@pytest.mark.gpu
+def test_round_limits_are_bounded(gpu):
+    src, dst, w = S.rmat(8, 4, seed=1, threads=1)
+    with S.DeviceGraph(src, dst, w, 256) as g:
+        for name, bad in (("max_rounds", 0), ("max_rounds", 2.0 ** 28 + 1), ("max_rounds", 2.0 ** 31),
+                          ("collect_budget", 0)):
+            with pytest.raises(S.SSSPError) as e:
+                g.set_param(name, bad)
+            assert e.value.code == S.ERR_ARG
+        g.set_param("max_rounds", 2 ** 28)
+        d, _ = g.solve(0, "push")
+        assert g.get_param("max_rounds") == 2 ** 28
 
 
 # ---- BFS on the same engine (BFS::gpu_bfs, algorithm/bfs/bfs.hpp:213-253) -----
@@ -533,3 +559,19 @@ def test_dense_rounds_with_split_blocks(gpu, oracle):
             assert np.array_equal(du, oracle.dijkstra(src, dst, wu, V, s)), s
             _, lev, _ = g.bfs(s, "all")
             assert np.array_equal(lev, oracle.bfs(src, dst, V, s)[1]), s
+
+
+@pytest.mark.gpu
+def test_python_mirror_cache_follows_in_place_edits(gpu, oracle):
+    """SingleSourceShortestPaths.gpu_bellman_ford caches the device graph; an
+    edit of the graph's arrays in place is seen by the next call."""
+    g = S.GraphGenerationAPI.R_MAT_parallel(1 << 11, 8, 57, 19, 19, 5, 8, True, 4)
+    op = S.SingleSourceShortestPaths()
+    d = np.empty(g.vertices_count, np.float32)
+    op.gpu_bellman_ford(g, d, 0)
+    assert np.array_equal(bits(d), bits(oracle.dijkstra(g.src_ids, g.dst_ids, g.weights,
+                                                          g.vertices_count, 0)))
+    g.weights[g.src_ids == 0] = 0.0  # in place
+    op.gpu_bellman_ford(g, d, 0)
+    assert np.array_equal(bits(d), bits(oracle.dijkstra(g.src_ids, g.dst_ids, g.weights,
+                                                          g.vertices_count, 0)))

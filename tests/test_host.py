@@ -173,3 +173,24 @@ def test_python_mirror_fails_loudly_without_library(monkeypatch):
     monkeypatch.setattr(S, "LIB_PATH", "/nonexistent/libsssp_mi355x.so")
     with pytest.raises(S.SSSPError):
         S.lib()
+
+
+def test_edge_fingerprint_tracks_edits_not_order():
+    """The Python mirror's device-graph cache key: the same edges in another
+    order give the same fingerprint (same distances), an edited weight or id
+    does not."""
+    s, d, w = S.rmat(10, 8, seed=3, threads=4)
+    f0 = S.edge_fingerprint(s, d, w)
+    p = np.random.default_rng(0).permutation(len(s))
+    assert S.edge_fingerprint(s[p], d[p], w[p]) == f0
+    w2 = w.copy()
+    w2[17] = np.nextafter(w2[17], np.float32(2))
+    assert S.edge_fingerprint(s, d, w2) != f0
+    d2 = d.copy()
+    d2[5] = (d2[5] + 1) % 1024
+    assert S.edge_fingerprint(s, d2, w) != f0
+    # swapping the weights of two edges changes the graph, and the fingerprint
+    w3 = w.copy()
+    i, j = 0, 1 + int(np.flatnonzero(w[1:] != w[0])[0])
+    w3[i], w3[j] = w[j], w[i]
+    assert S.edge_fingerprint(s, d, w3) != f0

@@ -140,7 +140,7 @@ def _brute(V, src, dst, w, source, dtype):
     adj = {}
     for a, b, c in zip(src, dst, w):
         adj.setdefault(int(a), []).append((int(b), c))
-    inf = FMAX if dtype == np.float32 else np.uint32(0xFFFFFFFF)
+    inf = {np.float32: FMAX, np.float64: np.finfo(np.float64).max}.get(dtype, np.uint32(0xFFFFFFFF))
     best = np.full(V, inf, dtype)
     best[source] = 0
 
@@ -148,7 +148,10 @@ def _brute(V, src, dst, w, source, dtype):
         for v, c in adj.get(u, []):
             if v in seen:
                 continue
-            nd = dtype(dist + c) if dtype == np.float32 else min(int(dist) + int(c), 0xFFFFFFFF)
+            if dtype == np.uint32:
+                nd = min(int(dist) + int(c), 0xFFFFFFFF)
+            else:
+                nd = dtype(dist + c)  # one IEEE add in the weight's own precision
             nd = dtype(nd)
             if nd < best[v]:
                 best[v] = nd
@@ -175,6 +178,83 @@ def test_oracle_equals_brute_force_tiny(oracle, seed):
     for source in range(3):
         d, _ = oracle.bf(src, dst, wu, V, source)
         assert np.array_equal(d, _brute(V, src, dst, wu, source, np.uint32))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_f64_equals_brute_force_tiny(oracle, seed):
+    """Double weights: the minimum over all simple paths of the sum taken in
+    double, with a zero weight and a weight far below an ulp of the sums (a
+    loop run in float, or with a dropped term, gives a different answer)."""
+    rng = np.random.default_rng(100 + seed)
+    V, E = 7, 16
+    src = rng.integers(0, V, E).astype(np.int32)
+    dst = rng.integers(0, V, E).astype(np.int32)
+    w = rng.random(E)
+    w[rng.integers(0, E, 2)] = 0.0
+    w[1] = 1e-300
+    for source in range(3):
+        d, _ = oracle.bf(src, dst, w, V, source)
+        exp = _brute(V, src, dst, w, source, np.float64)
+        assert np.array_equal(d.view(np.uint64), exp.view(np.uint64))
+        # rounding the weights to float changes the answer: the test resolves
+        # double precision
+        if source == 0:
+            df, _ = oracle.bf(src, dst, w.astype(np.float32), V, source)
+            reach = df < FMAX
+            assert not np.array_equal(df[reach].astype(np.float64), d[reach]) or reach.sum() == 1
+
+
+@pytest.mark.parametrize("V,ef", [(256, 4), (1024, 8)])
+def test_oracle_bf_f64_matches_reference_vectors(oracle, V, ef):
+    """cpu_bellman_ford<int, double> as the reference computes it (compiled
+    from its own sources into the golden vectors), bit for bit, DBL_MAX on
+    unreached vertices."""
+    g = gold(f"rmat_v{V}_ef{ef}_f64.npz")
+    for k, src in enumerate(g["sources"]):
+        d, _ = oracle.bf(g["src"], g["dst"], g["w"], V, int(src), threads=4)
+        assert np.array_equal(d.view(np.uint64), g["dist"][k].view(np.uint64))
+        assert (g["dist"][k] == np.finfo(np.float64).max).any()
+
+
+def test_oracle_f64_against_compiled_reference(oracle, ref_lib):
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    s, d, _ = S.rmat(12, 16, seed=9, threads=8)
+    w = np.random.default_rng(3).random(len(s))
+    V = 1 << 12
+    for src in (0, 33, 2000):
+        r = np.empty(V, np.float64)
+        assert ref_lib.ref_cpu_bellman_ford_f64(P(s), P(d), P(w), V, len(s), src, 4, P(r)) == 0
+        o, _ = oracle.bf(s, d, w, V, src)
+        assert np.array_equal(o.view(np.uint64), r.view(np.uint64))
+
+
+def test_oracle_capped_loop(oracle):
+    """The pass-capped loop behind the CPU baseline: with a cap at or above
+    the passes the full solve needs it is the full solve (same distances,
+    converged); below it, it stops after exactly `cap` passes, reports no
+    convergence, and its distances bound the fixpoint from above, each finite
+    one set through an in-edge whose source has only come down since
+    (d[u] + w <= d[v] for some in-edge u -> v with d[u] finite)."""
+    s, d, w = S.rmat(12, 8, seed=12, threads=8)
+    V = 1 << 12
+    full, passes = oracle.bf(s, d, w, V, 0, threads=1)
+    assert passes >= 3
+    for cap in (passes, passes + 5):
+        got, it, conv = oracle.bf_capped(s, d, w, V, 0, cap, threads=1)
+        assert conv == 1 and it == passes and np.array_equal(bits(got), bits(full))
+    for cap in (1, passes - 1):
+        got, it, conv = oracle.bf_capped(s, d, w, V, 0, cap, threads=1)
+        assert conv == 0 and it == cap
+        assert np.all(got >= full)
+        fin = np.flatnonzero((got < FMAX) & (np.arange(V) != 0))
+        below = np.zeros(V, bool)
+        ok = got[s] < FMAX
+        cand = (got[s[ok]] + w[ok]).astype(np.float32)
+        below[d[ok][cand <= got[d[ok]]]] = True
+        assert below[fin].all()
+    # one pass from the source relaxes at least its out-edges in edge order
+    got, _, _ = oracle.bf_capped(s, d, w, V, 0, 1, threads=1)
+    assert np.all(got[d[s == 0]] <= w[s == 0])
 
 
 @pytest.mark.parametrize("name,V,src,dst,w", small_graphs(), ids=[g[0] for g in small_graphs()])
