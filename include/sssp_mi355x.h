@@ -270,6 +270,18 @@ int sssp_bellman_ford_f64(const int32_t *d_src, const int32_t *d_dst, const doub
 int sssp_bellman_ford_u32(const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
                           int32_t vertices, int64_t edges, int32_t source, uint32_t *d_dist,
                           int32_t *passes);
+/* Allocation-free sssp_bellman_ford_f32: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * checks, passes and results as sssp_bellman_ford_f32, on the current device;
+ * it calls no hipMalloc/hipFree. Its scratch (64 change flags and 2 validation
+ * counters, 272 of 512 bytes) is a file-scope device array that each device
+ * keeps until the process ends; the scratch does not grow with the graph, so
+ * every size sssp_bellman_ford_f32 accepts is accepted. Calls on one device
+ * share that array: calls that overlap on one device race. */
+int sssp_bellman_ford_noalloc_f32(const int32_t *d_src, const int32_t *d_dst, const float *d_w,
+                                  int32_t vertices, int64_t edges, int32_t source, float *d_dist,
+                                  int32_t *passes);
 
 /* ---- host services ------------------------------------------------------- */
 /* GraphGenerationAPI::R_MAT_parallel (include/test/generation/

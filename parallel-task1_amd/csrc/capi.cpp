@@ -650,6 +650,19 @@ int sssp_bellman_ford_f32(const int32_t *d_src, const int32_t *d_dst, const floa
     });
 }
 
+int sssp_bellman_ford_noalloc_f32(const int32_t *d_src, const int32_t *d_dst, const float *d_w,
+                                  int32_t vertices, int64_t edges, int32_t source, float *d_dist,
+                                  int32_t *passes)
+{
+    return guard([&] {
+        SSSP_REQUIRE(d_dist && vertices >= 1 && edges >= 0 && (edges == 0 || (d_src && d_dst && d_w)),
+                     "bad arguments");
+        SSSP_REQUIRE(source >= 0 && source < vertices, "source vertex out of range");
+        ref_bf_noalloc_f32(d_src, d_dst, reinterpret_cast<const uint32_t *>(d_w), vertices, edges,
+                           reinterpret_cast<uint32_t *>(d_dist), passes);
+    });
+}
+
 int sssp_bellman_ford_u32(const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
                           int32_t vertices, int64_t edges, int32_t source, uint32_t *d_dist,
                           int32_t *passes)

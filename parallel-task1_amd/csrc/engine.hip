@@ -26,6 +26,7 @@
 // src-sorted edge list, with the convergence flag tested on the device.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "mlp_dev.hpp"
 #include "sssp_graph.hpp"
@@ -1995,19 +1996,21 @@ void ref_init(DT *d, int32_t V, int32_t source)
 
 template <class WT, class DT>
 void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int64_t E, DT *d,
-            int32_t *passes_out)
+            int32_t *passes_out, int32_t *changed_buf)
 {
     int dev = 0;
     SSSP_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     SSSP_HIP(hipGetDeviceProperties(&prop, dev));
-    int32_t *changed = dmalloc<int32_t>(64);
+    int32_t *changed = changed_buf ? changed_buf : dmalloc<int32_t>(64);
     int32_t hc[64];
     struct Free {
         void *p;
         ~Free() { dfree(p); }
-    } fr{changed};
-    SSSP_HIP(hipMemset(changed, 0, sizeof(int32_t) * 64));
+    } fr{changed_buf ? nullptr : changed};
+    // pass 0 reads no flag and sets flag 0 only on a change: all 64 start at 0
+    k_fill<int32_t><<<1, 64>>>(changed, 64, 0);
+    SSSP_LAUNCH_CHECK("clear change flags");
     int64_t nblocks = (int64_t)prop.multiProcessorCount * 8;
     const bool vec = sizeof(DT) == 4 && (reinterpret_cast<uintptr_t>(src) % 16 == 0) &&
                      (reinterpret_cast<uintptr_t>(dst) % 16 == 0) &&
@@ -2072,15 +2075,66 @@ void ref_bfs(const int32_t *src, const int32_t *dst, int32_t V, int64_t E, uint8
     SSSP_HIP(hipDeviceSynchronize());
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free f32 wrapper: the 64-slot ring of change flags
+// (bytes 0-255) and the two validation counters (bytes 256-271), in a
+// file-scope device array instead of memory allocated per call. The scratch
+// does not grow with the graph, so every size the allocating wrapper accepts
+// fits. Each device holds its own copy of the array at its own address, looked
+// up once per device (with that device current) by hipGetSymbolAddress; host
+// code passes that address to kernels and never the array's name, which in
+// host code only stands in for the symbol. Calls on one device share its
+// array: calls that overlap on one device race on the flags.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 512;
+constexpr size_t kScratchFlags = 0;  // int32_t[64]
+constexpr size_t kScratchBad = 256;  // int64_t[2]
+static_assert(kScratchBad + 2 * sizeof(int64_t) <= kNoAllocScratchBytes, "scratch layout");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kScratchDevices];
+
+static unsigned char *scratch_on_current_device()
+{
+    int dev = 0;
+    SSSP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        throw Error{SSSP_ERR_ARG, "device ordinal " + std::to_string(dev) +
+                                      " beyond the allocation-free wrapper's table of " +
+                                      std::to_string(kScratchDevices)};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error{SSSP_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                          hipGetErrorString(e)};
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+void ref_bf_noalloc_f32(const int32_t *src, const int32_t *dst, const uint32_t *w, int32_t V,
+                        int64_t E, uint32_t *d, int32_t *passes_out)
+{
+    unsigned char *p = scratch_on_current_device();
+    validate_edges(src, dst, w, SSSP_W_F32, E, V, nullptr,
+                   reinterpret_cast<int64_t *>(p + kScratchBad));
+    ref_bf<WF32, uint32_t>(src, dst, w, V, E, d, passes_out,
+                           reinterpret_cast<int32_t *>(p + kScratchFlags));
+}
+
 template void ref_init<WF32, uint32_t>(uint32_t *, int32_t, int32_t);
 template void ref_init<WU32, uint32_t>(uint32_t *, int32_t, int32_t);
 template void ref_init<WF64, unsigned long long>(unsigned long long *, int32_t, int32_t);
 template void ref_bf<WF32, uint32_t>(const int32_t *, const int32_t *, const uint32_t *, int32_t,
-                                     int64_t, uint32_t *, int32_t *);
+                                     int64_t, uint32_t *, int32_t *, int32_t *);
 template void ref_bf<WU32, uint32_t>(const int32_t *, const int32_t *, const uint32_t *, int32_t,
-                                     int64_t, uint32_t *, int32_t *);
+                                     int64_t, uint32_t *, int32_t *, int32_t *);
 template void ref_bf<WF64, unsigned long long>(const int32_t *, const int32_t *,
                                                const unsigned long long *, int32_t, int64_t,
-                                               unsigned long long *, int32_t *);
+                                               unsigned long long *, int32_t *, int32_t *);
 
 }  // namespace sssp

@@ -78,6 +78,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int64_t *in, int64_t
 // --------------------------------------------------------------------------
 // Input validation: ids in [0, V), weights >= 0 and not NaN.
 // --------------------------------------------------------------------------
+__global__ void k_zero_i64(int64_t *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 template <class W>
 __global__ __launch_bounds__(kBlock) void k_validate(const int32_t *src, const int32_t *dst,
                                                      const W *w, int64_t E, int32_t V,
@@ -540,17 +545,20 @@ void build_multi_layout(sssp_graph *g)
 // Edge-list check on the device: every id in [0, V) and (wtype >= 0) every
 // weight >= 0 and not NaN (unsigned weights always are); throws SSSP_ERR_ARG.
 // The relaxations rely on it: distances are compared as unsigned bit patterns,
-// which orders non-negative IEEE values only.
+// which orders non-negative IEEE values only. The two counters live in
+// `bad_buf` when the caller brings one (device memory, 16 bytes), else in a
+// buffer allocated for the call; a kernel clears them either way.
 void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int wtype, int64_t E,
-                    int32_t V, hipStream_t s)
+                    int32_t V, hipStream_t s, int64_t *bad_buf)
 {
     if (E <= 0) return;
-    int64_t *bad = dmalloc<int64_t>(2);
+    int64_t *bad = bad_buf ? bad_buf : dmalloc<int64_t>(2);
     struct Free {
         void *p;
         ~Free() { dfree(p); }
-    } fr{bad};
-    SSSP_HIP(hipMemsetAsync(bad, 0, 16, s));
+    } fr{bad_buf ? nullptr : bad};
+    k_zero_i64<<<1, 64, 0, s>>>(bad, 2);
+    SSSP_LAUNCH_CHECK("k_zero_i64");
     if (wtype == SSSP_W_F32)
         k_validate<float><<<grid_for(E), kBlock, 0, s>>>(src, dst, static_cast<const float *>(w), E,
                                                          V, bad);

@@ -266,9 +266,10 @@ void rmat_device(int scale, int ef, int a, int b, int c, int d, uint32_t base_se
 
 // layout.hip
 // ids in [0, V), and for wtype SSSP_W_F32/F64/U32 weights >= 0 and not NaN
-// (any other wtype: ids only); throws SSSP_ERR_ARG
+// (any other wtype: ids only); throws SSSP_ERR_ARG. bad_buf: 2 device
+// counters to use, or null (allocated for the call)
 void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int wtype, int64_t E,
-                    int32_t V, hipStream_t s);
+                    int32_t V, hipStream_t s, int64_t *bad_buf = nullptr);
 void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s);
 // packed 12-byte binary edge-list records (device) -> src, dst, w (device)
 void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, uint32_t *w,
@@ -286,9 +287,14 @@ void build_tile_table(const int64_t *scan, int32_t n, int64_t total, int32_t til
 // engine.hip
 template <class WT, class DT>
 void ref_init(DT *d, int32_t V, int32_t source);
+// changed_buf: the 64 change flags (device), or null (allocated for the call)
 template <class WT, class DT>
 void ref_bf(const int32_t *src, const int32_t *dst, const DT *w, int32_t V, int64_t E, DT *d,
-            int32_t *passes_out);
+            int32_t *passes_out, int32_t *changed_buf = nullptr);
+// bellman_ford_wrapper<float> with its scratch in a file-scope device array:
+// no device allocation (sssp_bellman_ford_noalloc_f32)
+void ref_bf_noalloc_f32(const int32_t *src, const int32_t *dst, const uint32_t *w, int32_t V,
+                        int64_t E, uint32_t *d, int32_t *passes_out);
 // BFS on the graph's edges (weights ignored): hop levels in g->dist, any schedule
 void bfs(sssp_graph *g, int32_t source, int sched, sssp_stats *st);
 void certify_levels(sssp_graph *g, int32_t source, int64_t *counts);

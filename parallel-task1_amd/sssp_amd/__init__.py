@@ -93,6 +93,7 @@ _SIGS = {
     "sssp_bfs_edges": (I32, [P, P, I32, I64, P, I32, PI32]),
     "sssp_bellman_ford_f64": (I32, [P, P, P, I32, I64, I32, P, PI32]),
     "sssp_bellman_ford_u32": (I32, [P, P, P, I32, I64, I32, P, PI32]),
+    "sssp_bellman_ford_noalloc_f32": (I32, [P, P, P, I32, I64, I32, P, PI32]),
     "sssp_rmat_generate": (I32, [I32, I32, I32, I32, I32, I32, U32, I32, P, P, P]),
     "sssp_er_generate": (I32, [I32, I32, U32, I32, P, P, P]),
     "sssp_rmat_generate_many": (I32, [I32, P, I32, I32, I32, I32, I32, U32, P, P, P, P]),
