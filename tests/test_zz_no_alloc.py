@@ -1,7 +1,7 @@
 """The allocation-free f32 wrapper, sssp_bellman_ford_noalloc_f32, against the
 allocating one it varies, sssp_bellman_ford_f32 (bellman_ford_wrapper<float>,
 algorithm/sssp/gpu/bellman_ford.cu:103-144), through the C ABI on caller-owned
-device arrays: the same distance bits and pass counts on two calls in a row,
+device arrays: the same distance bits on two calls in a row,
 the second on a smaller graph, and on the oracle's fixpoint. Its scratch (a
 file-scope device array shared by every call on the device) keeps the last
 call's contents, so the checks also run after a call that the validation
@@ -36,7 +36,10 @@ def test_noalloc_wrapper_matches_the_allocating_one(gpu, lib, oracle):
         assert rc == S.OK, S.last_error()
         rc, exp, p_exp = _wrapper(lib, "sssp_bellman_ford_f32", s, d, w, V, source)
         assert rc == S.OK, S.last_error()
-        assert np.array_equal(bits(got), bits(exp)) and p_got == p_exp >= 1
+        # the pass count is not compared: within a pass, how far updates
+        # propagate depends on the order the workgroups run in
+        assert np.array_equal(bits(got), bits(exp))
+        assert 1 <= p_got <= V + 4 and 1 <= p_exp <= V + 4
         assert np.array_equal(bits(got), bits(oracle.bf(s, d, w, V, source)[0]))
 
 
