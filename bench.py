@@ -170,7 +170,7 @@ def pmc_child(args):
 def bench_sources(g, args):
     """The bench sources: pick_sources depends only on which vertices have
     out-edges, read here from the device CSR."""
-    deg = np.diff(g.layout(csc=False)["row_ptr"])
+    deg = g.out_degrees()
     return S.pick_sources(np.nonzero(deg)[0].astype(np.int32), g.V, args.seed, args.sources)
 
 

@@ -65,6 +65,7 @@ extern "C" {
 /* ---- graph creation flags ------------------------------------------------ */
 #define SSSP_INPUT_DEVICE 1u   /* src/dst/w are device pointers on `device` */
 #define SSSP_NO_CSC       2u   /* skip the CSC (pull) layout */
+#define SSSP_NO_RELABEL   4u   /* keep vertex ids as the layouts' order (no degree-ordered slots) */
 
 typedef struct sssp_graph sssp_graph;
 typedef struct sssp_pack  sssp_pack;
@@ -139,8 +140,16 @@ int sssp_graph_create_from_bin(const char *path, int32_t directed, int32_t weigh
 int sssp_graph_destroy(sssp_graph *g);
 int sssp_graph_info(const sssp_graph *g, int32_t *vertices, int64_t *edges,
                     int32_t *weight_type, int64_t *device_bytes);
+/* The order the device layouts use: slot_of_vertex[v] (host, V elements) =
+ * the slot of vertex v. Unless SSSP_NO_RELABEL was given at creation, slots
+ * rank the vertices by descending in- plus out-degree (ties by id), so that
+ * the distances most edges read share cache lines; every argument and result
+ * of this API stays in the caller's vertex ids. No counterpart in the
+ * reference (its layouts keep the ids). */
+int sssp_graph_vertex_order(const sssp_graph *g, int32_t *slot_of_vertex);
 /* Copies the device layouts back (tests): row_ptr[V+1], sorted src/dst/w[E];
- * col_ptr[V+1], csc_src[E], csc_w[E] (csc arrays may be NULL). */
+ * col_ptr[V+1], csc_src[E], csc_w[E] (csc arrays may be NULL). The layouts
+ * index vertices by slot (see sssp_graph_vertex_order). */
 int sssp_graph_download_layout(const sssp_graph *g, int64_t *row_ptr, int32_t *el_src,
                                int32_t *el_dst, void *el_w, int64_t *col_ptr,
                                int32_t *csc_src, void *csc_w);

@@ -311,7 +311,24 @@ double batch_sources(sssp_graph *g, const int32_t *sources, int32_t n, const Con
         for (int32_t i = 0; i < n; i++) tasks[(size_t)i] = BatchTask{0, (int64_t)i * V, (int32_t)V, sources[i]};
         upload_tasks(g, tasks, work);
         // sources sorted by work are all equal here: order is preserved
-        return run_wg(g, n, V, c.bs, c.wgpcu);
+        double ms = run_wg(g, n, V, c.bs, c.wgpcu);
+        if (g->vperm) {
+            // rows to the caller's vertex order, one row at a time through
+            // dout (timed with the solve)
+            hipStream_t s = g->stream;
+            SSSP_HIP(hipEventRecord(g->ev_a, s));
+            for (int32_t i = 0; i < n; i++) {
+                uint32_t *row = g->bdist + (int64_t)i * V;
+                unpermute_rows(row, g->vperm, V, 1, g->dout, s);
+                SSSP_HIP(hipMemcpyAsync(row, g->dout, 4 * (size_t)V, hipMemcpyDeviceToDevice, s));
+            }
+            SSSP_HIP(hipEventRecord(g->ev_b, s));
+            SSSP_HIP(hipEventSynchronize(g->ev_b));
+            float extra = 0.f;
+            SSSP_HIP(hipEventElapsedTime(&extra, g->ev_a, g->ev_b));
+            ms += extra;
+        }
+        return ms;
     }
     ensure_batch_buffers(g, (int64_t)n * V, 0, 0, 0);
     const double saved = g->push_edge_frac;
@@ -322,7 +339,7 @@ double batch_sources(sssp_graph *g, const int32_t *sources, int32_t n, const Con
     for (int32_t i = 0; i < n; i++) {
         sssp_stats st{};
         solve(g, &sources[i], 1, SSSP_SCHED_HEURISTIC, &st);
-        SSSP_HIP(hipMemcpyAsync(g->bdist + (int64_t)i * V, g->dist, 4 * (size_t)V,
+        SSSP_HIP(hipMemcpyAsync(g->bdist + (int64_t)i * V, g->dout, 4 * (size_t)V,
                                 hipMemcpyDeviceToDevice, g->stream));
         total += st.t_solve_ms;
     }
@@ -412,9 +429,10 @@ __global__ __launch_bounds__(kBlock) void k_multi_init(uint32_t *D, int32_t n, i
     }
 }
 
-// out[s * V + v] = D[v * n + s] through 64 x 64 LDS tiles
+// out[s * V + v] = D[slot(v) * n + s] through 64 x 64 LDS tiles (slot(v) =
+// vperm[v] on a relabelled graph, else v): per source, caller vertex order
 __global__ __launch_bounds__(kBlock) void k_multi_out(const uint32_t *D, int32_t n, int64_t V,
-                                                      uint32_t *out)
+                                                      const int32_t *vperm, uint32_t *out)
 {
     __shared__ uint32_t tl[64][65];
     const int64_t v0 = (int64_t)blockIdx.x * 64;
@@ -423,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_multi_out(const uint32_t *D, int32_t
     for (int r = ty; r < 64; r += 4) {
         const int64_t v = v0 + r;
         const int s = s0 + tx;
-        if (v < V && s < n) tl[r][tx] = D[v * n + s];
+        if (v < V && s < n) tl[r][tx] = D[(vperm ? (int64_t)vperm[v] : v) * n + s];
     }
     __syncthreads();
     for (int r = ty; r < 64; r += 4) {
@@ -483,7 +501,7 @@ double run_multi(sssp_graph *g, const int32_t *sources, int32_t n)
                 throw Error{SSSP_ERR_NOT_CONVERGED, "multi-source passes exceeded V passes"};
         }
         k_multi_out<<<dim3((unsigned)((V + 63) / 64), (unsigned)((m + 63) / 64)), kBlock, 0, s>>>(
-            g->mdist, m, V, g->bdist + (int64_t)c0 * V);
+            g->mdist, m, V, g->vperm, g->bdist + (int64_t)c0 * V);
         SSSP_LAUNCH_CHECK("k_multi_out");
         SSSP_HIP(hipEventRecord(g->ev_b, s));
         SSSP_HIP(hipEventSynchronize(g->ev_b));
@@ -607,9 +625,16 @@ double pack_run(sssp_pack *p, const int32_t *local_sources, const Config &c)
     g->push_edge_frac = saved;
     g->push_chunk = saved_chunk;
     ensure_batch_buffers(g, g->V, 0, 0, 0);
-    SSSP_HIP(hipMemcpyAsync(g->bdist, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToDevice, g->stream));
+    SSSP_HIP(hipMemcpyAsync(g->bdist, g->dout, 4 * (size_t)g->V, hipMemcpyDeviceToDevice, g->stream));
     SSSP_HIP(hipStreamSynchronize(g->stream));
     return st.t_solve_ms;
+}
+
+std::vector<int32_t> source_slots(const sssp_graph *g, const int32_t *sources, int32_t n)
+{
+    std::vector<int32_t> out((size_t)n);
+    for (int32_t i = 0; i < n; i++) out[(size_t)i] = slot_of(g, sources[i]);
+    return out;
 }
 
 void check_pack_sources(const sssp_pack *p, const int32_t *sources)
@@ -643,7 +668,8 @@ int sssp_solve_batch(sssp_graph *g, const int32_t *sources, int32_t n, int32_t p
             c = (policy == 1)   ? Config{1, 1024, 1, 0.05, 1}
                 : (policy == 3) ? Config{3, kMThreads, 1, 0.0, 0}
                                 : Config{2, 256, 2, g->push_edge_frac, g->push_chunk};
-        const double ms = batch_sources(g, sources, n, c);
+        const std::vector<int32_t> slots = source_slots(g, sources, n);
+        const double ms = batch_sources(g, slots.data(), n, c);
         if (dist_out)
             SSSP_HIP(hipMemcpy(dist_out, g->bdist, 4 * (size_t)n * (size_t)g->V, hipMemcpyDeviceToHost));
         fill_stats(stats, ms, (double)g->E * n);
@@ -659,7 +685,8 @@ int sssp_autotune_batch(sssp_graph *g, const int32_t *sources, int32_t n, const 
         for (int32_t i = 0; i < n; i++)
             SSSP_REQUIRE(sources[i] >= 0 && sources[i] < g->V, "source vertex out of range");
         SSSP_HIP(hipSetDevice(g->device));
-        const Config b = autotune([&](const Config &c) { return batch_sources(g, sources, n, c); },
+        const std::vector<int32_t> slots = source_slots(g, sources, n);
+        const Config b = autotune([&](const Config &c) { return batch_sources(g, slots.data(), n, c); },
                                   tuning_file, best_ms, "batch of sources", true);
         g->batch_policy = b.policy;
         g->batch_bs = b.bs;
@@ -705,8 +732,10 @@ int sssp_pack_create(const int32_t *src, const int32_t *dst, const float *weight
             }
         }
         SSSP_REQUIRE(bad == 0, "pack: edge endpoint outside its graph");
+        // the union keeps its ids: a graph's vertices stay one contiguous
+        // range, which the workgroup-per-graph policy relies on
         const int rc2 = sssp_graph_create(SSSP_W_F32, us.data(), ud.data(), weights, (int32_t)V, E,
-                                          device, 0, &p->g);
+                                          device, SSSP_NO_RELABEL, &p->g);
         if (rc2 != SSSP_OK) throw Error{rc2, sssp_last_error()};
         *out = p;
     });

@@ -44,7 +44,8 @@ void graph_free(sssp_graph *g)
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
                     g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap,
-                    g->b_src, g->b_dst, g->b_w, g->b_tile, g->m_rec, g->m_tile, g->mdist};
+                    g->b_src, g->b_dst, g->b_w, g->b_tile, g->m_rec, g->m_tile, g->mdist,
+                    g->vperm, g->vinv, g->dout != g->dist ? g->dout : nullptr};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {
@@ -165,9 +166,11 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
                   uint32_t flags)
 {
     hipStream_t s = g->stream;
+    g->relabel = !(flags & SSSP_NO_RELABEL);
     build_layouts(g, d_src, d_dst, d_w, !(flags & SSSP_NO_CSC));
     const size_t V = (size_t)g->V;
     g->dist = dmalloc<uint32_t>(V);
+    g->dout = g->relabel ? dmalloc<uint32_t>(V) : g->dist;
     g->stamp = dmalloc<int32_t>(V);
     g->pile[0] = dmalloc<int32_t>(V);
     g->pile[1] = dmalloc<int32_t>(V);
@@ -208,7 +211,7 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     const int64_t xe = g->xgoff_host.empty() ? 0 : g->xgoff_host.back();
     g->device_bytes = Ei * 12 + xe * 12 + g->b_edges * 10 + (int64_t)g->n_btile * 24 +
                       (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) + Vi * 4 * 7 +
-                      (Vi + 1) * 4 * 3 + (Vi + 1) * 8 * 2 + Vi * 24;
+                      (Vi + 1) * 4 * 3 + (Vi + 1) * 8 * 2 + Vi * 24 + (g->relabel ? Vi * 12 : 0);
 }
 
 struct DevBuf {
@@ -399,6 +402,14 @@ int sssp_graph_info(const sssp_graph *g, int32_t *vertices, int64_t *edges, int3
     });
 }
 
+int sssp_graph_vertex_order(const sssp_graph *g, int32_t *slot_of_vertex)
+{
+    return guard([&] {
+        SSSP_REQUIRE(g && slot_of_vertex, "null argument");
+        for (int32_t v = 0; v < g->V; v++) slot_of_vertex[v] = slot_of(g, v);
+    });
+}
+
 int sssp_graph_download_layout(const sssp_graph *g, int64_t *row_ptr, int32_t *el_src,
                                int32_t *el_dst, void *el_w, int64_t *col_ptr, int32_t *csc_src,
                                void *csc_w)
@@ -431,12 +442,13 @@ int sssp_solve(sssp_graph *g, int32_t source, int32_t schedule, void *dist_out, 
         std::memset(&st, 0, sizeof(st));
         g->have_result = false;
         g->bfs_result = false;
-        solve(g, &source, 1, schedule, &st);
+        const int32_t slot = slot_of(g, source);
+        solve(g, &slot, 1, schedule, &st);
         g->last_source = source;
         g->last_sched = schedule;
         g->have_result = true;
         if (dist_out)
-            SSSP_HIP(hipMemcpy(dist_out, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+            SSSP_HIP(hipMemcpy(dist_out, g->dout, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
         if (stats) *stats = st;
     });
 }
@@ -453,17 +465,17 @@ int sssp_bfs(sssp_graph *g, int32_t source, int32_t schedule, uint8_t *visited_o
         std::memset(&st, 0, sizeof(st));
         g->have_result = false;
         g->bfs_result = false;
-        bfs(g, source, schedule, &st);
+        bfs(g, slot_of(g, source), schedule, &st);
         g->last_source = source;
         g->last_sched = schedule;
         g->bfs_result = true;
         if (levels_out)
-            SSSP_HIP(hipMemcpy(levels_out, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+            SSSP_HIP(hipMemcpy(levels_out, g->dout, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
         if (visited_out) {
             uint8_t *dv = dmalloc<uint8_t>((size_t)g->V);
             DevBuf own;
             own.p[0] = dv;
-            levels_to_visited(g->dist, g->V, dv, g->stream);
+            levels_to_visited(g->dout, g->V, dv, g->stream);
             SSSP_HIP(hipStreamSynchronize(g->stream));
             SSSP_HIP(hipMemcpy(visited_out, dv, (size_t)g->V, hipMemcpyDeviceToHost));
         }
@@ -489,7 +501,7 @@ int sssp_get_distances(sssp_graph *g, void *dist_out)
         SSSP_REQUIRE(g && dist_out, "null argument");
         SSSP_REQUIRE(g->have_result, "no solve result on this graph");
         set_device(g->device);
-        SSSP_HIP(hipMemcpy(dist_out, g->dist, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+        SSSP_HIP(hipMemcpy(dist_out, g->dout, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
     });
 }
 
@@ -518,10 +530,10 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
         SSSP_REQUIRE(g->have_result || g->bfs_result, "no solve result on this graph");
         SSSP_REQUIRE(source >= 0 && source < g->V, "source vertex out of range");
         set_device(g->device);
-        if (g->bfs_result) certify_levels(g, source, counts);  // hop levels: every edge weighs 1
-        else certify(g, &source, 1, counts);
-        if (parent_out)
-            SSSP_HIP(hipMemcpy(parent_out, g->parent, 4 * (size_t)g->V, hipMemcpyDeviceToHost));
+        const int32_t slot = slot_of(g, source);
+        if (g->bfs_result) certify_levels(g, slot, counts);  // hop levels: every edge weighs 1
+        else certify(g, &slot, 1, counts);
+        if (parent_out) parents_out(g, parent_out);
         if (valid) *valid = (counts[0] == 0 && counts[1] == 0 && counts[2] == 0) ? 1 : 0;
     });
 }
@@ -701,7 +713,7 @@ int sssp_collect_timings(sssp_graph *g, int32_t source, int32_t cap, float *feat
         set_device(g->device);
         g->have_result = false;
         g->bfs_result = false;
-        collect_timings(g, source, cap, features, times_ms, rows);
+        collect_timings(g, slot_of(g, source), cap, features, times_ms, rows);
         g->last_source = source;
         if (g->collect_kind == 1) g->bfs_result = true;  // the collected solve was a BFS
         else g->have_result = true;

@@ -1159,6 +1159,34 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
 }
 
 // ---------------------------------------------------------------------------
+// Caller order from slot order (relabelled graphs): a gather of each output
+// row through the vertex order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_unpermute(const uint32_t *in, const int32_t *vperm,
+                                                      int64_t V, int64_t rows, uint32_t *out)
+{
+    const int64_t n = V * rows;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int64_t r = i / V, v = i - r * V;
+        out[i] = in[r * V + vperm[v]];
+    }
+}
+
+// parent_out[v] = caller id of the parent of v's slot (-1 stays -1)
+__global__ __launch_bounds__(kBlock) void k_unpermute_parents(const int32_t *par,
+                                                              const int32_t *vperm,
+                                                              const int32_t *vinv, int64_t V,
+                                                              int32_t *out)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < V;
+         v += (int64_t)gridDim.x * kBlock) {
+        const int32_t p = par[vperm[v]];
+        out[v] = p < 0 ? p : vinv[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Certificate helpers.
 // ---------------------------------------------------------------------------
 template <class WT>
@@ -1686,6 +1714,8 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
         g->pile_dirty = false;
         g->tick = tick0 + st->iterations + 2;
     }
+    // the result in the caller's vertex order is part of the solve: timed
+    if (g->vperm) unpermute_rows(g->dist, g->vperm, g->V, 1, g->dout, s);
     SSSP_HIP(hipEventRecord(g->ev_b, s));
     SSSP_HIP(hipEventSynchronize(g->ev_b));
     float ms = 0.f;
@@ -1935,6 +1965,33 @@ void reset_stamps(sssp_graph *g)
     k_fill<int32_t><<<grid_for(g->V), kBlock, 0, g->stream>>>(g->farst, g->V, 0);
     SSSP_LAUNCH_CHECK("pile state reset");
     g->pile_dirty = false;
+}
+
+void unpermute_rows(const uint32_t *in, const int32_t *vperm, int64_t V, int64_t rows,
+                    uint32_t *out, hipStream_t s)
+{
+    if (V * rows == 0) return;
+    k_unpermute<<<grid_for(V * rows), kBlock, 0, s>>>(in, vperm, V, rows, out);
+    SSSP_LAUNCH_CHECK("k_unpermute");
+}
+
+void parents_out(sssp_graph *g, int32_t *parent_host)
+{
+    hipStream_t s = g->stream;
+    if (!g->vperm) {
+        SSSP_HIP(hipMemcpyAsync(parent_host, g->parent, 4 * (size_t)g->V, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    int32_t *tmp = dmalloc<int32_t>((size_t)g->V);
+    struct Free {
+        void *p;
+        ~Free() { dfree(p); }
+    } fr{tmp};
+    k_unpermute_parents<<<grid_for(g->V), kBlock, 0, s>>>(g->parent, g->vperm, g->vinv, g->V, tmp);
+    SSSP_LAUNCH_CHECK("k_unpermute_parents");
+    SSSP_HIP(hipMemcpyAsync(parent_host, tmp, 4 * (size_t)g->V, hipMemcpyDeviceToHost, s));
+    SSSP_HIP(hipStreamSynchronize(s));
 }
 
 void solve(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, sssp_stats *st)

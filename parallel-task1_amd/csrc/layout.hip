@@ -78,6 +78,57 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int64_t *in, int64_t
 // --------------------------------------------------------------------------
 // Input validation: ids in [0, V), weights >= 0 and not NaN.
 // --------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Vertex order (slots by descending in- plus out-degree).
+// ---------------------------------------------------------------------------
+// tot[v] = in- plus out-degree, out[v] = out-degree
+__global__ __launch_bounds__(kBlock) void k_degrees(const int32_t *src, const int32_t *dst,
+                                                    int64_t E, uint32_t *tot, uint32_t *out)
+{
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * kBlock) {
+        const int32_t s = src[e];
+        atomicAdd(&tot[s], 1u);
+        atomicAdd(&out[s], 1u);
+        atomicAdd(&tot[dst[e]], 1u);
+    }
+}
+
+// ascending sort key = descending count (counts above kOrderCap tie), of
+// vertex at[i] (or of vertex i when `at` is null)
+constexpr uint32_t kOrderCap = 1u << 30;
+__global__ __launch_bounds__(kBlock) void k_order_keys(const uint32_t *cnt, const uint32_t *at,
+                                                       int64_t V, uint32_t *key)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < V;
+         i += (int64_t)gridDim.x * kBlock)
+        key[i] = kOrderCap - min(cnt[at ? at[i] : i], kOrderCap);
+}
+
+// slot i holds vertex by_out[pos[i]] -> vinv, and its inverse vperm
+__global__ __launch_bounds__(kBlock) void k_invert_order(const uint32_t *by_out,
+                                                         const uint32_t *pos, int64_t V,
+                                                         int32_t *vperm, int32_t *vinv)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < V;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int32_t v = (int32_t)by_out[pos[i]];
+        vinv[i] = v;
+        vperm[v] = (int32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_relabel_edges(const int32_t *src, const int32_t *dst,
+                                                          int64_t E, const int32_t *vperm,
+                                                          int32_t *osrc, int32_t *odst)
+{
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * kBlock) {
+        osrc[e] = vperm[src[e]];
+        odst[e] = vperm[dst[e]];
+    }
+}
+
 __global__ void k_zero_i64(int64_t *p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
@@ -588,23 +639,89 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
 
     validate_edges(d_src, d_dst, d_w, g->wtype, E, V, s);
 
-    // scratch for the sorts
-    const int64_t rs_tiles = (E + kRsTile - 1) / kRsTile;
+    // scratch for the sorts (of E edges, and of V vertices for the order)
+    const int64_t n_sort = std::max<int64_t>(E, (int64_t)V);
+    const int64_t rs_tiles = (n_sort + kRsTile - 1) / kRsTile;
     const int64_t scan_n = std::max<int64_t>((int64_t)V + 1, 256 * rs_tiles);
     g->n_blk = scan_blocks_needed(scan_n);
     g->blk_a = dmalloc<int64_t>((size_t)g->n_blk);
     g->blk_b = dmalloc<int64_t>((size_t)g->n_blk);
-    uint32_t *kA = dmalloc<uint32_t>((size_t)E), *kB = dmalloc<uint32_t>((size_t)E);
-    uint32_t *vA = dmalloc<uint32_t>((size_t)E), *vB = dmalloc<uint32_t>((size_t)E);
+    uint32_t *kA = dmalloc<uint32_t>((size_t)n_sort), *kB = dmalloc<uint32_t>((size_t)n_sort);
+    uint32_t *vA = dmalloc<uint32_t>((size_t)n_sort), *vB = dmalloc<uint32_t>((size_t)n_sort);
     int64_t *counts = dmalloc<int64_t>((size_t)std::max<int64_t>(256 * rs_tiles, (int64_t)V + 1));
     int64_t *tmp_pos = dmalloc<int64_t>((size_t)V + 1);
     struct Scratch {
-        void *p[6];
+        void *p[8];
         ~Scratch()
         {
             for (void *x : p) dfree(x);
         }
-    } scratch{{kA, kB, vA, vB, counts, tmp_pos}};
+    } scratch{{kA, kB, vA, vB, counts, tmp_pos, nullptr, nullptr}};
+
+    // ---- vertex order: slot = rank by descending in- plus out-degree, ties
+    // by descending out-degree, then by vertex id (two stable sorts: by
+    // out-degree, then by total degree); the edges are renamed into slots
+    // before any layout is built. On R-MAT graphs the hubs are vertex ids
+    // with few one bits, scattered over the whole id range, so the distance
+    // gathers of a dense round touch a cache line per hub; in slot order the
+    // hubs' distances are one dense prefix, and among the many vertices of
+    // equal small degree the more often gathered come first. Measured at
+    // R-MAT scale 25 (dense round per launch, learned solve): ids 2.04 /
+    // 21.3 ms; total degree, ties by id 1.70 / 20.0; out-degree 1.79 / 20.5;
+    // in-degree 1.68 / 20.1; total degree, ties by out-degree 1.61 / 19.0.
+    if (g->relabel && E > 0 && V > 1) {
+        uint32_t *tot = dmalloc<uint32_t>((size_t)V);
+        scratch.p[6] = tot;
+        uint32_t *outd = dmalloc<uint32_t>((size_t)V);
+        scratch.p[7] = outd;
+        SSSP_HIP(hipMemsetAsync(tot, 0, 4 * (size_t)V, s));
+        SSSP_HIP(hipMemsetAsync(outd, 0, 4 * (size_t)V, s));
+        k_degrees<<<grid_for(E), kBlock, 0, s>>>(d_src, d_dst, E, tot, outd);
+        SSSP_LAUNCH_CHECK("k_degrees");
+        uint32_t *okeys = dmalloc<uint32_t>((size_t)V);
+        uint32_t *by_out = dmalloc<uint32_t>((size_t)V);
+        struct F0 {
+            void *p[2];
+            ~F0()
+            {
+                for (void *x : p) dfree(x);
+            }
+        } f0{{okeys, by_out}};
+        uint32_t *sk = nullptr, *ord = nullptr;
+        k_order_keys<<<grid_for(V), kBlock, 0, s>>>(outd, nullptr, V, okeys);
+        SSSP_LAUNCH_CHECK("k_order_keys out");
+        radix_sort_pairs(reinterpret_cast<const int32_t *>(okeys), V, (int32_t)kOrderCap, kA, kB,
+                         vA, vB, counts, g->blk_a, g->blk_b, g->n_blk, s, &sk, &ord);
+        SSSP_HIP(hipMemcpyAsync(by_out, ord, 4 * (size_t)V, hipMemcpyDeviceToDevice, s));
+        k_order_keys<<<grid_for(V), kBlock, 0, s>>>(tot, by_out, V, okeys);
+        SSSP_LAUNCH_CHECK("k_order_keys total");
+        radix_sort_pairs(reinterpret_cast<const int32_t *>(okeys), V, (int32_t)kOrderCap, kA, kB,
+                         vA, vB, counts, g->blk_a, g->blk_b, g->n_blk, s, &sk, &ord);
+        g->vperm = dmalloc<int32_t>((size_t)V);
+        g->vinv = dmalloc<int32_t>((size_t)V);
+        k_invert_order<<<grid_for(V), kBlock, 0, s>>>(by_out, ord, V, g->vperm, g->vinv);
+        SSSP_LAUNCH_CHECK("k_invert_order");
+        // the renamed edges take the degree arrays' places in the scratch
+        // list: both are dead once the order exists
+        SSSP_HIP(hipStreamSynchronize(s));
+        dfree(tot);
+        dfree(outd);
+        scratch.p[6] = scratch.p[7] = nullptr;
+        int32_t *rs = dmalloc<int32_t>((size_t)E);
+        scratch.p[6] = rs;
+        int32_t *rd = dmalloc<int32_t>((size_t)E);
+        scratch.p[7] = rd;
+        k_relabel_edges<<<grid_for(E), kBlock, 0, s>>>(d_src, d_dst, E, g->vperm, rs, rd);
+        SSSP_LAUNCH_CHECK("k_relabel_edges");
+        g->vperm_host.resize((size_t)V);
+        SSSP_HIP(hipMemcpyAsync(g->vperm_host.data(), g->vperm, 4 * (size_t)V,
+                                hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        d_src = rs;
+        d_dst = rd;
+    } else {
+        g->relabel = false;
+    }
 
     // ---- CSR / src-sorted edge list ----
     uint32_t *skeys = nullptr, *perm = nullptr;

@@ -155,6 +155,19 @@ struct sssp_graph {
     int32_t *sample_v = nullptr;
     int32_t n_sample = 0;
 
+    // vertex order: the layouts and every per-vertex array index vertices by
+    // slot, slots in descending order of in- plus out-degree (ties by vertex
+    // id), so that the distances of the vertices most edges read share cache
+    // lines. vperm[v] = slot of caller vertex v, vinv[slot] = caller vertex
+    // (both null, and slot = vertex, when the graph is not relabelled)
+    bool relabel = false;
+    int32_t *vperm = nullptr;
+    int32_t *vinv = nullptr;
+    std::vector<int32_t> vperm_host;
+    // distances (or BFS levels) of the last solve in caller order: a buffer
+    // of its own when relabelled, else the same memory as dist
+    uint32_t *dout = nullptr;
+
     // solve state
     uint32_t *dist = nullptr;
     int32_t *stamp = nullptr;
@@ -236,6 +249,12 @@ struct sssp_graph {
 
 namespace sssp {
 
+// Slot of caller vertex v (the identity on a graph that is not relabelled).
+inline int32_t slot_of(const sssp_graph *g, int32_t v)
+{
+    return g->vperm_host.empty() ? v : g->vperm_host[(size_t)v];
+}
+
 // Block-wide exclusive scan of one int64 per thread (kBlock threads);
 // s_warp must hold kBlock/64 entries. Contains barriers: call uniformly.
 __device__ inline int64_t block_excl_scan_i64(int64_t x, int64_t *total, int64_t *s_warp)
@@ -285,6 +304,12 @@ void build_tile_table(const int64_t *scan, int32_t n, int64_t total, int32_t til
                       int32_t *tile_out, hipStream_t s);
 
 // engine.hip
+// out[r * V + v] = in[r * V + vperm[v]] for rows r < rows: caller order from
+// slot order (in and out must not overlap)
+void unpermute_rows(const uint32_t *in, const int32_t *vperm, int64_t V, int64_t rows,
+                    uint32_t *out, hipStream_t s);
+// the certificate's parent tree of the last certify call, in caller ids, to host
+void parents_out(sssp_graph *g, int32_t *parent_host);
 template <class WT, class DT>
 void ref_init(DT *d, int32_t V, int32_t source);
 // changed_buf: the 64 change flags (device), or null (allocated for the call)

@@ -28,7 +28,7 @@ OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_INTERNAL, ERR_IO, ERR_NOT_CONVERGED = range(7
 W_F32, W_U32, W_F64 = 0, 1, 2
 SCHED_PLAIN, SCHED_HEURISTIC, SCHED_LEARNED, SCHED_PUSH, SCHED_ALL, SCHED_PULL = range(6)
 MODE_PUSH, MODE_ALL, MODE_PULL, MODE_VERIFY, MODE_DENSE = 0, 1, 2, 3, 4
-INPUT_DEVICE, NO_CSC = 1, 2
+INPUT_DEVICE, NO_CSC, NO_RELABEL = 1, 2, 4
 MLP_IN, MLP_HID, MLP_MODES, MLP_CHUNKS = 5, 64, 3, 4
 MLP_OUT = MLP_MODES + MLP_CHUNKS
 MLP_NPARAM = MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_OUT + MLP_OUT
@@ -73,6 +73,7 @@ _SIGS = {
     "sssp_graph_destroy": (I32, [P]),
     "sssp_graph_info": (I32, [P, PI32, PI64, PI32, PI64]),
     "sssp_graph_download_layout": (I32, [P, P, P, P, P, P, P, P]),
+    "sssp_graph_vertex_order": (I32, [P, P]),
     "sssp_solve": (I32, [P, I32, I32, P, C.POINTER(Stats)]),
     "sssp_get_distances": (I32, [P, P]),
     "sssp_get_trace": (I32, [P, I32, P, P, P, P, PI32]),
@@ -237,12 +238,16 @@ def features(fv, fe, unsettled, rnd, mean_deg, V, E):
 # --------------------------------------------------------------------------
 # device graph
 # --------------------------------------------------------------------------
+def _flags(csc, relabel):
+    return (0 if csc else NO_CSC) | (0 if relabel else NO_RELABEL)
+
+
 class DeviceGraph:
     """A graph uploaded once to HBM (edge list sorted by src, CSR, CSC)."""
 
     @classmethod
     def rmat(cls, scale, edge_factor, a=57, b=19, c=19, d=5, seed=1, threads=64, device=0,
-             csc=True, download_edges=False):
+             csc=True, download_edges=False, relabel=True):
         """Generates the R-MAT graph on the device (bit-identical to rmat())
         and builds it there; optionally returns the generated edge list."""
         self = cls.__new__(cls)
@@ -251,7 +256,7 @@ class DeviceGraph:
             if download_edges else (None, None, None)
         h = C.c_void_p()
         _check(lib().sssp_graph_create_rmat(scale, edge_factor, a, b, c, d, seed & 0xFFFFFFFF,
-                                            threads, device, 0 if csc else NO_CSC,
+                                            threads, device, _flags(csc, relabel),
                                             _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]),
                                             C.byref(h)))
         self._h = h
@@ -261,7 +266,8 @@ class DeviceGraph:
         return (self, arrs) if download_edges else self
 
     @classmethod
-    def from_bin(cls, path, directed=True, weight_dtype=np.float32, device=0, csc=True):
+    def from_bin(cls, path, directed=True, weight_dtype=np.float32, device=0, csc=True,
+                 relabel=True):
         """Device ingest of a binary edge-list file (the reference's format):
         chunks stream through pinned buffers onto the device, where the
         layouts are built; no host copy of the graph is made."""
@@ -274,7 +280,7 @@ class DeviceGraph:
             raise SSSPError(ERR_ARG, f"unsupported weight dtype {weight_dtype}")
         h = C.c_void_p()
         _check(lib().sssp_graph_create_from_bin(str(path).encode(), 1 if directed else 0,
-                                                self.wtype, device, 0 if csc else NO_CSC,
+                                                self.wtype, device, _flags(csc, relabel),
                                                 C.byref(h)))
         self._h = h
         self.device = device
@@ -283,7 +289,8 @@ class DeviceGraph:
         self.V, self.E = V.value, E.value
         return self
 
-    def __init__(self, src, dst, w, vertices: int, device: int = 0, csc: bool = True):
+    def __init__(self, src, dst, w, vertices: int, device: int = 0, csc: bool = True,
+                 relabel: bool = True):
         w = np.ascontiguousarray(w)
         if w.dtype == np.float32:
             self.wtype, self.wdtype = W_F32, np.float32
@@ -298,7 +305,7 @@ class DeviceGraph:
         self.V, self.E = int(vertices), len(src)
         h = C.c_void_p()
         _check(lib().sssp_graph_create(self.wtype, _ptr(src), _ptr(dst), _ptr(w), self.V, self.E,
-                                       device, 0 if csc else NO_CSC, C.byref(h)))
+                                       device, _flags(csc, relabel), C.byref(h)))
         self._h = h
         self.device = device
 
@@ -373,7 +380,20 @@ class DeviceGraph:
                                   counts.ctypes.data_as(PI64), C.byref(valid)))
         return bool(valid.value), counts, par
 
+    def vertex_order(self):
+        """slot[v]: the position of vertex v in the device layouts (the
+        identity when the graph was built with relabel=False)."""
+        o = np.empty(self.V, np.int32)
+        _check(lib().sssp_graph_vertex_order(self._h, _ptr(o)))
+        return o
+
+    def out_degrees(self):
+        """Out-degree of every vertex, in the caller's vertex ids."""
+        deg = np.diff(self.layout(csc=False)["row_ptr"])
+        return deg[self.vertex_order()]
+
     def layout(self, csc=True):
+        """The device layouts; vertices are numbered by slot (vertex_order)."""
         V1, E = self.V + 1, self.E
         rp = np.empty(V1, np.int64)
         s = np.empty(E, np.int32)

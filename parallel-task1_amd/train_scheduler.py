@@ -57,8 +57,7 @@ def collect(scales, sources_per_scale, edge_factor, seed0, net=None):
         seed = seed0 + k
         g = S.DeviceGraph.rmat(scale, edge_factor, seed=seed, threads=64)
         _rollout_with(g, net)
-        L = g.layout(csc=False)
-        deg = np.diff(L["row_ptr"])
+        deg = g.out_degrees()
         srcs = S.pick_sources(np.repeat(np.arange(g.V, dtype=np.int32), deg), g.V, seed,
                               sources_per_scale)
         for s in srcs:

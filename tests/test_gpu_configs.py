@@ -231,7 +231,7 @@ def test_c5_rmat25_learned_heuristic_plain(gpu, oracle):
         V, E = g.V, g.E
         assert V == 1 << 25 and E == 536870912
         m = _with_network(g)
-        deg = np.diff(g.layout(csc=False)["row_ptr"])
+        deg = g.out_degrees()
         sources = S.pick_sources(np.nonzero(deg)[0].astype(np.int32), V, 6, 8)
         t = {"learned": [], "heuristic": [], "plain": []}
         first = None

@@ -85,12 +85,26 @@ def test_device_ingest_from_bin(gpu, oracle, tmp_path):
 
 
 @pytest.mark.gpu
-def test_layout_is_stable_sort(gpu):
+@pytest.mark.parametrize("relabel", [True, False])
+def test_layout_is_stable_sort(gpu, relabel):
+    """The layouts are stable sorts of the edge list renamed into slots; slots
+    rank the vertices by descending in- plus out-degree, then descending
+    out-degree, then vertex id (the identity with relabel=False)."""
     src, dst, w = S.rmat(11, 16, seed=3, threads=4)
     V = 1 << 11
-    with S.DeviceGraph(src, dst, w, V) as g:
-        g.set_param("plain_below_edges", 0)  # exercise the round engine
+    with S.DeviceGraph(src, dst, w, V, relabel=relabel) as g:
         L = g.layout()
+        order = g.vertex_order()
+    if relabel:
+        out = np.bincount(src, minlength=V)
+        tot = out + np.bincount(dst, minlength=V)
+        exp = np.empty(V, np.int32)
+        exp[np.lexsort((np.arange(V), -out, -tot))] = np.arange(V, dtype=np.int32)
+        assert np.array_equal(order, exp)
+        assert not np.array_equal(order, np.arange(V))
+    else:
+        assert np.array_equal(order, np.arange(V))
+    src, dst = order[src], order[dst]
     p = np.argsort(src, kind="stable")
     assert np.array_equal(L["el_src"], src[p])
     assert np.array_equal(L["el_dst"], dst[p])
