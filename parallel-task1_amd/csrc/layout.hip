@@ -376,6 +376,11 @@ __global__ __launch_bounds__(kBlock) void k_m_gather(const uint32_t *skeys, cons
 }
 
 // row_ptr[v] = first position whose sorted key is >= v; row_ptr[V] = E.
+// Position i starts the vertices (keys[i-1], keys[i]]; it writes them when
+// they are few. A long run of keyless vertices (on a degree-ordered graph
+// every isolated vertex sits at the end) is left at -1 for
+// k_rowptr_search, so no thread loops over millions of vertices.
+constexpr int64_t kRowptrRun = 64;
 __global__ __launch_bounds__(kBlock) void k_rowptr_from_sorted(const uint32_t *keys, int64_t E,
                                                                int32_t V, int64_t *row_ptr)
 {
@@ -383,9 +388,35 @@ __global__ __launch_bounds__(kBlock) void k_rowptr_from_sorted(const uint32_t *k
          i += (int64_t)gridDim.x * kBlock) {
         int64_t lo = (i == 0) ? 0 : (int64_t)keys[i - 1] + 1;  // first vertex not yet placed
         int64_t hi = (i == E) ? (int64_t)V : (int64_t)keys[i];  // last vertex starting at i
-        for (int64_t v = lo; v <= hi; v++) row_ptr[v] = i;
+        if (hi - lo < kRowptrRun)
+            for (int64_t v = lo; v <= hi; v++) row_ptr[v] = i;
     }
 }
+
+// the entries k_rowptr_from_sorted left at -1: lower bound of v in the keys
+__global__ __launch_bounds__(kBlock) void k_rowptr_search(const uint32_t *keys, int64_t E,
+                                                          int32_t V, int64_t *row_ptr)
+{
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v <= V;
+         v += (int64_t)gridDim.x * kBlock) {
+        if (row_ptr[v] >= 0) continue;
+        int64_t lo = 0, hi = E;  // first position with key >= v
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if ((int64_t)keys[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        row_ptr[v] = lo;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i64(int64_t *p, int64_t n, int64_t val)
+{
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        p[i] = val;
+}
+
 
 __global__ __launch_bounds__(kBlock) void k_nz_flags(const int64_t *ptr, int32_t V, int64_t *flags)
 {
@@ -426,6 +457,15 @@ inline int grid_for(int64_t n, int64_t cap = 8192)
     int64_t b = (n + kBlock - 1) / kBlock;
     if (b < 1) b = 1;
     return (int)(b < cap ? b : cap);
+}
+
+void rowptr_from_sorted(const uint32_t *keys, int64_t E, int32_t V, int64_t *row_ptr,
+                        hipStream_t s)
+{
+    k_fill_i64<<<grid_for((int64_t)V + 1), kBlock, 0, s>>>(row_ptr, (int64_t)V + 1, -1);
+    k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(keys, E, V, row_ptr);
+    k_rowptr_search<<<grid_for((int64_t)V + 1), kBlock, 0, s>>>(keys, E, V, row_ptr);
+    SSSP_LAUNCH_CHECK("rowptr_from_sorted");
 }
 
 // Sorts by key; returns the permutation (original indices) in *perm_out
@@ -571,8 +611,7 @@ void build_multi_layout(sssp_graph *g)
     MRec *rec = dmalloc<MRec>((size_t)E);
     k_m_gather<<<grid_for(E), kBlock, 0, s>>>(skeys, perm, E, g->el_src, g->el_dst, g->el_w, rec);
     SSSP_LAUNCH_CHECK("k_m_gather");
-    k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(skeys, E, nblk, bptr);
-    SSSP_LAUNCH_CHECK("k_rowptr_from_sorted multi");
+    rowptr_from_sorted(skeys, E, nblk, bptr, s);
     std::vector<int64_t> bp((size_t)nblk + 1);
     SSSP_HIP(hipMemcpyAsync(bp.data(), bptr, 8 * bp.size(), hipMemcpyDeviceToHost, s));
     SSSP_HIP(hipStreamSynchronize(s));
@@ -736,8 +775,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         k_gather_pair<uint32_t><<<grid_for(E), kBlock, 0, s>>>(perm, E, d_dst, d_w, g->el_dst, g->el_w);
         SSSP_LAUNCH_CHECK("csr gather");
     }
-    k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(skeys, E, V, g->row_ptr);
-    SSSP_LAUNCH_CHECK("k_rowptr_from_sorted");
+    rowptr_from_sorted(skeys, E, V, g->row_ptr, s);
     build_static_list(g->row_ptr, V, E, counts, tmp_pos, g->blk_a, g->blk_b, g->n_blk, s,
                       &g->nzo_v, &g->nzo_scan, &g->n_nzo, g->nzo_tile);
 
@@ -808,8 +846,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         uint32_t *bkeys = nullptr, *bperm = nullptr;
         radix_sort_pairs(reinterpret_cast<const int32_t *>(keys), E, nblk - 1, kA, kB, vA, vB,
                          counts, g->blk_a, g->blk_b, g->n_blk, s, &bkeys, &bperm);
-        k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(bkeys, E, nblk, bptr);
-        SSSP_LAUNCH_CHECK("k_rowptr_from_sorted blk");
+        rowptr_from_sorted(bkeys, E, nblk, bptr, s);
         std::vector<int64_t> bp((size_t)nblk + 1), bo((size_t)nblk + 1);
         SSSP_HIP(hipMemcpyAsync(bp.data(), bptr, 8 * bp.size(), hipMemcpyDeviceToHost, s));
         SSSP_HIP(hipStreamSynchronize(s));
@@ -878,8 +915,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
             k_gather_pair<uint32_t><<<grid_for(E), kBlock, 0, s>>>(perm, E, d_src, d_w, g->csc_src, g->csc_w);
             SSSP_LAUNCH_CHECK("csc gather");
         }
-        k_rowptr_from_sorted<<<grid_for(E + 1), kBlock, 0, s>>>(skeys, E, V, g->col_ptr);
-        SSSP_LAUNCH_CHECK("k_rowptr_from_sorted csc");
+        rowptr_from_sorted(skeys, E, V, g->col_ptr, s);
         build_static_list(g->col_ptr, V, E, counts, tmp_pos, g->blk_a, g->blk_b, g->n_blk, s,
                           &g->nzi_v, &g->nzi_scan, &g->n_nzi, g->nzi_tile);
         g->has_csc = true;
