@@ -420,8 +420,8 @@ class DeviceGraphCache {
 }  // namespace sssp_host
 
 // SingleSourceShortestPaths<V,W>::gpu_bellman_ford (sssp.hpp:219-278): the
-// graph is uploaded once and cached (keyed on its data pointers and sizes),
-// the solve runs on the device, the distances are copied to `distances`, and
+// graph is uploaded once and cached (DeviceGraphCache: re-uploaded whenever the
+// Graph changed or its edges' fingerprint did), the solve runs on the device, the distances are copied to `distances`, and
 // the return value is MTEPS over the solve window (uploads and downloads
 // excluded, as the reference's t2..t3).
 template <typename _TVertexValue, typename _TEdgeWeight>
