@@ -179,7 +179,8 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
  * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
- * "trace_events"; "cu_count" and "collect_rounds" read-only).
+ * "batch_capture", "trace_events"; "cu_count", "collect_rounds" and
+ * "multi_passes" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
  * 2^14 destinations with twice the mean in-edges) and dense_unreached
  * (default 0.5): an all-edge round whose frontier has at least
@@ -204,6 +205,15 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.
  * max_rounds (default 100000) is bounded to [1, 2^28]; a solve that needs more
  * rounds returns SSSP_ERR_NOT_CONVERGED.
+ * The plain schedule queues its passes four at a time (the change flags are
+ * kept and tested on the device; the host reads them after each four); with
+ * trace_events = 1 each pass is launched alone between HIP events.
+ * batch_capture (default 1): a batch of policy 3 runs each chunk of up to 256
+ * sources as captured hipGraph launches -- init and a run of 32 passes (one
+ * launch each, the pass number read from device memory, passes after the
+ * fixpoint returning at once) and the transposed output -- replayed per call,
+ * one host check per run of 32 passes; 0 queues the passes from the host, four
+ * per check. "multi_passes" (read-only) = passes the last policy-3 batch ran.
  * Distances do not depend on any knob. */
 int sssp_set_param(sssp_graph *g, const char *name, double value);
 int sssp_get_param(const sssp_graph *g, const char *name, double *value);

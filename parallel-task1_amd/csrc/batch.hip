@@ -353,19 +353,17 @@ double batch_sources(sssp_graph *g, const int32_t *sources, int32_t n, const Con
 constexpr int kMThreads = 256;  // sources per chunk: one thread per source
 constexpr int kMBlock = 1 << kMBits;
 
+// One tile of a vertex-major pass: thread s holds source s's distances of
+// the tile's 16-vertex destination block in its LDS column, lowers them with
+// the tile's records (a record's source row gathered for all sources at
+// once), and writes back what it lowered. Returns whether this lane lowered a
+// distance.
 template <class WT>
-__global__ __launch_bounds__(kMThreads) void k_multi_pass(const BlkTile *__restrict__ tiles,
-                                                          const MRec *__restrict__ rec,
-                                                          uint32_t *D, int32_t n, int64_t V,
-                                                          int32_t *changed, int32_t pass)
+__device__ __forceinline__ bool multi_tile(const BlkTile &t, const MRec *__restrict__ rec,
+                                           uint32_t *D, int32_t n, int64_t V,
+                                           uint32_t (&ld)[kMBlock][kMThreads])
 {
     constexpr uint32_t INF = WT::INF;
-    if (blockIdx.x == 0 && threadIdx.x == 0) changed[(pass + 1) & 63] = 0;
-    if (pass > 0 && __hip_atomic_load(&changed[(pass - 1) & 63], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT) == 0)
-        return;
-    __shared__ uint32_t ld[kMBlock][kMThreads];
-    const BlkTile t = tiles[blockIdx.x];
     const int s = threadIdx.x;
     const bool act = s < n;
     const int64_t base = (int64_t)t.blk << kMBits;
@@ -412,6 +410,27 @@ __global__ __launch_bounds__(kMThreads) void k_multi_pass(const BlkTile *__restr
             }
         }
     }
+    return low;
+}
+
+// One pass per launch, one workgroup per tile. The pass number is base[0] +
+// k (base in device memory, so a captured run of passes is replayed as the
+// next run); pass p zeroes the flag pass p+1 sets and returns at once when
+// pass p-1 changed nothing.
+template <class WT>
+__global__ __launch_bounds__(kMThreads) void k_multi_pass(const BlkTile *__restrict__ tiles,
+                                                          const MRec *__restrict__ rec,
+                                                          uint32_t *D, int32_t n, int64_t V,
+                                                          int32_t *changed, const int32_t *base,
+                                                          int32_t k)
+{
+    const int32_t pass = *base + k;
+    if (blockIdx.x == 0 && threadIdx.x == 0) changed[(pass + 1) & 63] = 0;
+    if (pass > 0 && __hip_atomic_load(&changed[(pass - 1) & 63], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == 0)
+        return;
+    __shared__ uint32_t ld[kMBlock][kMThreads];
+    const bool low = multi_tile<WT>(tiles[blockIdx.x], rec, D, n, V, ld);
     if (__ballot(low) != 0 && __lane_id() == 0) changed[pass & 63] = 1;
 }
 
@@ -451,6 +470,127 @@ __global__ __launch_bounds__(kBlock) void k_multi_out(const uint32_t *D, int32_t
     }
 }
 
+__global__ void k_zero_words(int32_t *p, int32_t n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// base[0] += k (the next run of captured passes)
+__global__ void k_advance_base(int32_t *base, int32_t k)
+{
+    if (threadIdx.x == 0) base[0] += k;
+}
+
+constexpr int32_t kMultiRun = 32;  // passes per captured run (the flag ring has 64)
+
+template <class WT>
+void launch_multi_passes(sssp_graph *g, int32_t m, hipStream_t s)
+{
+    for (int32_t k = 0; k < kMultiRun; k++)
+        k_multi_pass<WT><<<(unsigned)g->n_mtile, kMThreads, 0, s>>>(
+            g->m_tile, g->m_rec, g->mdist, m, g->V, g->mflags, g->mflags + 64, k);
+}
+
+// The captured launches of one chunk of at most 256 sources, keyed on chunk
+// shape and buffers: `first` clears the change flags and the pass base with a
+// kernel, initialises the vertex-major distances, runs kMultiRun passes and
+// transposes the result into the per-source output; `more` advances the pass
+// base by kMultiRun and runs the next kMultiRun passes and the transpose.
+// Passes after the fixpoint return at once.
+struct MultiGraphs {
+    hipGraphExec_t first, more;
+};
+
+MultiGraphs multi_graphs(sssp_graph *g, int32_t c0, int32_t m)
+{
+    const int64_t V = g->V;
+    const int64_t key[kMultiKey] = {m,
+                                    c0,
+                                    V,
+                                    g->n_mtile,
+                                    (int64_t)(uintptr_t)g->mflags,
+                                    (int64_t)(uintptr_t)g->mdist,
+                                    (int64_t)(uintptr_t)g->bdist,
+                                    (int64_t)(uintptr_t)g->msrc,
+                                    g->wtype,
+                                    (int64_t)(uintptr_t)g->m_tile};
+    for (int i = 0; i < kMultiSlots; i++)
+        if (g->mexec[i][0] && std::memcmp(key, g->mkey[i], sizeof(key)) == 0)
+            return MultiGraphs{(hipGraphExec_t)g->mexec[i][0], (hipGraphExec_t)g->mexec[i][1]};
+    const int slot = g->mnext;
+    g->mnext = (g->mnext + 1) % kMultiSlots;
+    for (int j = 0; j < 2; j++)
+        if (g->mexec[slot][j]) {
+            (void)hipGraphExecDestroy((hipGraphExec_t)g->mexec[slot][j]);
+            g->mexec[slot][j] = nullptr;
+        }
+    hipStream_t s = g->stream;
+    SSSP_HIP(hipStreamSynchronize(s));
+    const bool u32 = g->wtype == SSSP_W_U32;
+    const unsigned gi = (unsigned)std::min<int64_t>((V * m + kBlock - 1) / kBlock, 65536);
+    for (int j = 0; j < 2; j++) {
+        hipGraph_t graph;
+        SSSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        if (j == 0) {
+            k_zero_words<<<1, 128, 0, s>>>(g->mflags, 65);
+            if (u32) k_multi_init<WU32><<<gi, kBlock, 0, s>>>(g->mdist, m, V, g->msrc + c0);
+            else k_multi_init<WF32><<<gi, kBlock, 0, s>>>(g->mdist, m, V, g->msrc + c0);
+        } else {
+            k_advance_base<<<1, 64, 0, s>>>(g->mflags + 64, kMultiRun);
+        }
+        if (g->n_mtile > 0) {
+            if (u32) launch_multi_passes<WU32>(g, m, s);
+            else launch_multi_passes<WF32>(g, m, s);
+        }
+        k_multi_out<<<dim3((unsigned)((V + 63) / 64), (unsigned)((m + 63) / 64)), kBlock, 0, s>>>(
+            g->mdist, m, V, g->vperm, g->bdist + (int64_t)c0 * V);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (ce == hipSuccess) (void)hipGraphDestroy(graph);
+            throw Error{SSSP_ERR_HIP, std::string("capturing the batch launch failed: ") +
+                                          hipGetErrorString(le != hipSuccess ? le : ce)};
+        }
+        hipGraphExec_t exec;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        SSSP_HIP(ie);
+        g->mexec[slot][j] = exec;
+    }
+    std::memcpy(g->mkey[slot], key, sizeof(key));
+    return MultiGraphs{(hipGraphExec_t)g->mexec[slot][0], (hipGraphExec_t)g->mexec[slot][1]};
+}
+
+// Passes of one chunk as queued launches (batch_capture = 0): four at a time,
+// the host reading the change flags after each four.
+int32_t multi_passes_launched(sssp_graph *g, int32_t m)
+{
+    hipStream_t s = g->stream;
+    const int64_t V = g->V;
+    int32_t *changed = g->mflags;
+    k_zero_words<<<1, 128, 0, s>>>(changed, 65);
+    int32_t pass = 0, done_at = -1;
+    while (done_at < 0 && g->n_mtile > 0) {
+        for (int i = 0; i < 4; i++) {
+            if (g->wtype == SSSP_W_U32)
+                k_multi_pass<WU32><<<(unsigned)g->n_mtile, kMThreads, 0, s>>>(
+                    g->m_tile, g->m_rec, g->mdist, m, V, changed, changed + 64, pass + i);
+            else
+                k_multi_pass<WF32><<<(unsigned)g->n_mtile, kMThreads, 0, s>>>(
+                    g->m_tile, g->m_rec, g->mdist, m, V, changed, changed + 64, pass + i);
+            SSSP_LAUNCH_CHECK("k_multi_pass");
+        }
+        SSSP_HIP(hipMemcpyAsync(g->ctrl_host->changed, changed, 4 * 64, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < 4 && done_at < 0; i++)
+            if (g->ctrl_host->changed[(pass + i) & 63] == 0) done_at = pass + i;
+        pass += 4;
+        if (done_at < 0 && pass > g->V + 4)
+            throw Error{SSSP_ERR_NOT_CONVERGED, "multi-source passes exceeded V passes"};
+    }
+    return done_at + 1;
+}
+
 double run_multi(sssp_graph *g, const int32_t *sources, int32_t n)
 {
     hipStream_t s = g->stream;
@@ -463,48 +603,59 @@ double run_multi(sssp_graph *g, const int32_t *sources, int32_t n)
         g->mdist = dmalloc<uint32_t>((size_t)chunk * (size_t)V);
         g->mdist_elems = (int64_t)chunk * V;
     }
-    int32_t *dsrc = dmalloc<int32_t>((size_t)n);
-    struct F {
-        void *p;
-        ~F() { dfree(p); }
-    } fsrc{dsrc};
-    SSSP_HIP(hipMemcpyAsync(dsrc, sources, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+    if (n > g->msrc_cap) {
+        dfree(g->msrc);
+        g->msrc = nullptr;
+        g->msrc = dmalloc<int32_t>((size_t)n);
+        g->msrc_cap = n;
+    }
+    if (!g->mflags) g->mflags = dmalloc<int32_t>(65);  // 64 change flags, the pass base
+    SSSP_HIP(hipMemcpyAsync(g->msrc, sources, 4 * (size_t)n, hipMemcpyHostToDevice, s));
     float total = 0.f;
+    g->multi_passes = 0;
     for (int32_t c0 = 0; c0 < n; c0 += kMThreads) {
         const int32_t m = std::min<int32_t>(kMThreads, n - c0);
-        SSSP_HIP(hipEventRecord(g->ev_a, s));
-        const unsigned gi = (unsigned)std::min<int64_t>((V * m + kBlock - 1) / kBlock, 65536);
-        if (g->wtype == SSSP_W_U32)
-            k_multi_init<WU32><<<gi, kBlock, 0, s>>>(g->mdist, m, V, dsrc + c0);
-        else
-            k_multi_init<WF32><<<gi, kBlock, 0, s>>>(g->mdist, m, V, dsrc + c0);
-        SSSP_LAUNCH_CHECK("k_multi_init");
-        int32_t *changed = g->ctrl->changed;
-        SSSP_HIP(hipMemsetAsync(changed, 0, 4 * 64, s));
-        int32_t pass = 0, done_at = -1;
-        while (done_at < 0 && g->n_mtile > 0) {
-            for (int i = 0; i < 4; i++) {
-                if (g->wtype == SSSP_W_U32)
-                    k_multi_pass<WU32><<<(unsigned)g->n_mtile, kMThreads, 0, s>>>(
-                        g->m_tile, g->m_rec, g->mdist, m, V, changed, pass + i);
-                else
-                    k_multi_pass<WF32><<<(unsigned)g->n_mtile, kMThreads, 0, s>>>(
-                        g->m_tile, g->m_rec, g->mdist, m, V, changed, pass + i);
-                SSSP_LAUNCH_CHECK("k_multi_pass");
+        if (g->batch_capture) {
+            const MultiGraphs mg = multi_graphs(g, c0, m);
+            SSSP_HIP(hipEventRecord(g->ev_a, s));
+            int32_t done_at = -1;
+            // replay until a run holds a pass that changed nothing (at most
+            // V + 1 passes, as the pass loop)
+            for (int32_t run = 0; done_at < 0; run++) {
+                if (g->n_mtile == 0) {
+                    SSSP_HIP(hipGraphLaunch(mg.first, s));
+                    done_at = 0;
+                    break;
+                }
+                if ((int64_t)run * kMultiRun > V + 1)
+                    throw Error{SSSP_ERR_NOT_CONVERGED, "multi-source passes exceeded V passes"};
+                SSSP_HIP(hipGraphLaunch(run == 0 ? mg.first : mg.more, s));
+                SSSP_HIP(hipMemcpyAsync(g->ctrl_host->changed, g->mflags, 4 * 64,
+                                        hipMemcpyDeviceToHost, s));
+                SSSP_HIP(hipStreamSynchronize(s));
+                for (int32_t i = 0; i < kMultiRun && done_at < 0; i++) {
+                    const int32_t p = run * kMultiRun + i;
+                    if (g->ctrl_host->changed[p & 63] == 0) done_at = p;
+                }
             }
-            SSSP_HIP(hipMemcpyAsync(g->ctrl_host->changed, changed, 4 * 64, hipMemcpyDeviceToHost, s));
-            SSSP_HIP(hipStreamSynchronize(s));
-            for (int i = 0; i < 4 && done_at < 0; i++)
-                if (g->ctrl_host->changed[(pass + i) & 63] == 0) done_at = pass + i;
-            pass += 4;
-            if (done_at < 0 && pass > g->V + 4)
-                throw Error{SSSP_ERR_NOT_CONVERGED, "multi-source passes exceeded V passes"};
+            SSSP_HIP(hipEventRecord(g->ev_b, s));
+            SSSP_HIP(hipEventSynchronize(g->ev_b));
+            g->multi_passes += done_at + 1;
+        } else {
+            SSSP_HIP(hipEventRecord(g->ev_a, s));
+            const unsigned gi = (unsigned)std::min<int64_t>((V * m + kBlock - 1) / kBlock, 65536);
+            if (g->wtype == SSSP_W_U32)
+                k_multi_init<WU32><<<gi, kBlock, 0, s>>>(g->mdist, m, V, g->msrc + c0);
+            else
+                k_multi_init<WF32><<<gi, kBlock, 0, s>>>(g->mdist, m, V, g->msrc + c0);
+            SSSP_LAUNCH_CHECK("k_multi_init");
+            g->multi_passes += multi_passes_launched(g, m);
+            k_multi_out<<<dim3((unsigned)((V + 63) / 64), (unsigned)((m + 63) / 64)), kBlock, 0,
+                          s>>>(g->mdist, m, V, g->vperm, g->bdist + (int64_t)c0 * V);
+            SSSP_LAUNCH_CHECK("k_multi_out");
+            SSSP_HIP(hipEventRecord(g->ev_b, s));
+            SSSP_HIP(hipEventSynchronize(g->ev_b));
         }
-        k_multi_out<<<dim3((unsigned)((V + 63) / 64), (unsigned)((m + 63) / 64)), kBlock, 0, s>>>(
-            g->mdist, m, V, g->vperm, g->bdist + (int64_t)c0 * V);
-        SSSP_LAUNCH_CHECK("k_multi_out");
-        SSSP_HIP(hipEventRecord(g->ev_b, s));
-        SSSP_HIP(hipEventSynchronize(g->ev_b));
         float ms = 0.f;
         SSSP_HIP(hipEventElapsedTime(&ms, g->ev_a, g->ev_b));
         total += ms;

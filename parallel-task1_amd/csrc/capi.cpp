@@ -45,8 +45,11 @@ void graph_free(sssp_graph *g)
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
                     g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap,
                     g->b_src, g->b_dst, g->b_w, g->b_tile, g->m_rec, g->m_tile, g->mdist,
-                    g->vperm, g->vinv, g->dout != g->dist ? g->dout : nullptr};
+                    g->vperm, g->vinv, g->dout != g->dist ? g->dout : nullptr, g->msrc, g->mflags};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
+    for (auto &pair : g->mexec)
+        for (void *e : pair)
+            if (e) (void)hipGraphExecDestroy((hipGraphExec_t)e);
     for (void *p : ptrs) dfree(p);
     for (int c = 0; c < kChunkCount; c++) {
         dfree(g->nzo_tile[c]);
@@ -576,6 +579,10 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "dense_kernel") {
             SSSP_REQUIRE(value == 0 || value == 1, "dense_kernel must be 0 or 1");
             g->dense_kernel = (int)value;
+        } else if (n == "batch_capture") {
+            SSSP_REQUIRE(value == 0 || value == 1, "batch_capture must be 0 or 1");
+            g->batch_capture = (int)value;
+
         } else if (n == "dense_unreached") {
             SSSP_REQUIRE(value >= 0 && value <= 2, "dense_unreached must be in [0, 2]");
             g->dense_unreached = value;
@@ -612,6 +619,9 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "dense_all_frac") *value = g->dense_all_frac;
         else if (n == "dense_unreached") *value = g->dense_unreached;
         else if (n == "dense_kernel") *value = g->dense_kernel;
+        else if (n == "batch_capture") *value = g->batch_capture;
+
+        else if (n == "multi_passes") *value = g->multi_passes;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;
         else if (n == "collect_budget") *value = (double)g->collect_budget;

@@ -78,6 +78,8 @@ struct BlkTile {
 // of distances is 16 KB of LDS; tiles of at most kMTileEdges edges of one
 // block.
 constexpr int kMBits = 4;
+constexpr int kMultiSlots = 4;  // captured chunk launches kept per graph
+constexpr int kMultiKey = 10;
 constexpr int64_t kMTileEdges = 4096;
 struct MRec {
     int32_t src;
@@ -135,6 +137,16 @@ struct sssp_graph {
     int32_t n_mtile = 0;
     uint32_t *mdist = nullptr;
     int64_t mdist_elems = 0;
+    // the batch's sources on the device, and the captured launches of its
+    // chunks (hipGraphExec_t), keyed on chunk shape and buffers
+    int32_t *msrc = nullptr;
+    int32_t msrc_cap = 0;
+    int32_t *mflags = nullptr;  // 64 change flags of the passes, then the pass base
+    void *mexec[sssp::kMultiSlots][2] = {};  // first run, next runs
+    int64_t mkey[sssp::kMultiSlots][sssp::kMultiKey] = {};
+    int mnext = 0;
+    int batch_capture = 1;     // policy 3: 1 = captured launches (hipGraph) per chunk
+    int32_t multi_passes = 0;  // passes the last policy-3 batch ran (over its chunks)
     // CSC: in-edges grouped by dst
     int64_t *col_ptr = nullptr;
     int32_t *csc_src = nullptr;

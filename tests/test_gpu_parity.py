@@ -243,6 +243,63 @@ def test_batch_sources_match_single_solves(gpu, oracle, policy):
 
 
 @pytest.mark.gpu
+def test_batch_captured_launch_replays(gpu, oracle):
+    """Policy 3 runs each chunk of sources as captured hipGraph launches (change
+    flags and pass base cleared by a kernel, init, runs of 32 passes reading
+    their pass number from the device, transposed output). Three replays with
+    other sources, eager solves between them: each bit-equal to the passes
+    queued from the host and to policy 1 (a workgroup per source)."""
+    src, dst, w = S.rmat(14, 16, seed=41, threads=16)
+    V = 1 << 14
+    with S.DeviceGraph(src, dst, w, V) as g:
+        for k in range(3):
+            sources = S.pick_sources(src, V, 41 + k, 64)
+            print(f"captured batch replay {k}: 64 sources, RMAT-14", flush=True)
+            g.set_param("batch_capture", 1)
+            D, _ = g.solve_batch(sources, policy=3)
+            assert g.get_param("multi_passes") >= 1
+            d, _ = g.solve(int(sources[k]), "heuristic")
+            assert np.array_equal(bits(D[k]), bits(d))
+            g.set_param("batch_capture", 0)
+            D0, _ = g.solve_batch(sources, policy=3)
+            assert np.array_equal(bits(D), bits(D0))
+            D1, _ = g.solve_batch(sources, policy=1)
+            assert np.array_equal(bits(D), bits(D1))
+        ref, _ = oracle.bf(src, dst, w, V, int(sources[5]))
+        assert np.array_equal(bits(D[5]), bits(ref))
+
+
+@pytest.mark.gpu
+def test_plain_solve_queued_and_traced(gpu, oracle):
+    """The plain schedule queues its passes four at a time (change flags tested
+    on the device); with trace_events each pass is launched alone between
+    events. Same distances; the traced solve records one row per pass, the
+    last one a pass that changed nothing; a pass cap below the fixpoint gives
+    up and leaves the graph usable."""
+    src, dst, w = S.rmat(15, 16, seed=42, threads=16)
+    V = 1 << 15
+    ref = oracle.dijkstra(src, dst, w, V, 7)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        d, st = g.solve(7, "plain")
+        assert np.array_equal(bits(d), bits(ref))
+        assert 2 <= st["iterations"] <= V + 1 and st["verify_passes"] == 1
+        g.set_param("trace_events", 1)
+        d2, st2 = g.solve(7, "plain")
+        g.set_param("trace_events", 0)
+        assert np.array_equal(bits(d2), bits(ref))
+        tr = g.trace()
+        assert len(tr["ms"]) == st2["iterations"] and np.all(tr["ms"] > 0)
+        # a plain solve capped below its pass count gives up, then solves again
+        g.set_param("max_rounds", 1)
+        with pytest.raises(S.SSSPError) as e:
+            g.solve(7, "plain")
+        assert e.value.code == S.ERR_NOT_CONVERGED
+        g.set_param("max_rounds", 100000)
+        d3, _ = g.solve(7, "plain")
+        assert np.array_equal(bits(d3), bits(ref))
+
+
+@pytest.mark.gpu
 def test_batch_u32(gpu, oracle):
     src, dst, w = S.er(1 << 12, 8, seed=4, weight_mode=1)
     V = 1 << 12
