@@ -179,8 +179,9 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "advance_blocks", "verify", "max_rounds", "push_edge_frac", "pull_frac",
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
  * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
- * "batch_capture", "trace_events"; "cu_count", "collect_rounds" and
- * "multi_passes" read-only).
+ * "batch_capture", "decide_spin", "blk_tile_cap", "blk_tile_order",
+ * "trace_events"; "cu_count", "collect_rounds", "multi_passes" and
+ * "blk_tiles" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
  * 2^14 destinations with twice the mean in-edges) and dense_unreached
  * (default 0.5): an all-edge round whose frontier has at least
@@ -214,6 +215,13 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * fixpoint returning at once) and the transposed output -- replayed per call,
  * one host check per run of 32 passes; 0 queues the passes from the host, four
  * per check. "multi_passes" (read-only) = passes the last policy-3 batch ran.
+ * blk_tile_cap / blk_tile_order rebuild the tile table of the dense rounds'
+ * destination-blocked list: at most blk_tile_cap edges per tile (a multiple
+ * of 4; default 2^17, doubled while 4096 tiles of the cap fit in E) and the
+ * dispatch order 0 = smallest tiles first (default), 1 = by block, 2 = largest
+ * first, 3 = by descending block. decide_spin (default 1): the host polls the
+ * mapped control block for each round's decision instead of synchronizing
+ * the stream.
  * Distances do not depend on any knob. */
 int sssp_set_param(sssp_graph *g, const char *name, double value);
 int sssp_get_param(const sssp_graph *g, const char *name, double *value);

@@ -1,5 +1,6 @@
 // capi.cpp -- the extern "C" boundary of libsssp_mi355x.so: handle lifetime,
 // uploads, status codes, and the reference-shaped wrappers.
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -579,6 +580,17 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "dense_kernel") {
             SSSP_REQUIRE(value == 0 || value == 1, "dense_kernel must be 0 or 1");
             g->dense_kernel = (int)value;
+        } else if (n == "blk_tile_cap") {
+            SSSP_REQUIRE(value >= 256 && value <= 2147483648.0 && std::fmod(value, 4.0) == 0,
+                         "blk_tile_cap must be a multiple of 4 in [256, 2^31]");
+            retile_blocked(g, (int64_t)value, g->blk_tile_order);
+        } else if (n == "blk_tile_order") {
+            SSSP_REQUIRE(value == 0 || value == 1 || value == 2 || value == 3,
+                         "blk_tile_order must be 0, 1, 2 or 3");
+            retile_blocked(g, g->blk_tile_cap, (int)value);
+        } else if (n == "decide_spin") {
+            SSSP_REQUIRE(value == 0 || value == 1, "decide_spin must be 0 or 1");
+            g->decide_spin = (int)value;
         } else if (n == "batch_capture") {
             SSSP_REQUIRE(value == 0 || value == 1, "batch_capture must be 0 or 1");
             g->batch_capture = (int)value;
@@ -620,6 +632,10 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "dense_unreached") *value = g->dense_unreached;
         else if (n == "dense_kernel") *value = g->dense_kernel;
         else if (n == "batch_capture") *value = g->batch_capture;
+        else if (n == "decide_spin") *value = g->decide_spin;
+        else if (n == "blk_tile_cap") *value = (double)g->blk_tile_cap;
+        else if (n == "blk_tile_order") *value = g->blk_tile_order;
+        else if (n == "blk_tiles") *value = g->n_btile;
 
         else if (n == "multi_passes") *value = g->multi_passes;
         else if (n == "rollout_policy") *value = g->rollout_policy;

@@ -25,6 +25,7 @@
 // The plain schedule (SSSP_SCHED_PLAIN) is exactly that loop over the
 // src-sorted edge list, with the convergence flag tested on the device.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 
@@ -817,6 +818,7 @@ struct DecideArgs {
     const float *mlp;  // nullable
     int pile;          // pile buffer the round reads
     Ctrl *mirror;      // device address of the host mirror (coherent host memory)
+    int32_t seq;       // written to mirror->dec_seq once the mirror holds this round
 };
 
 // One wave: features, the hand-written rule and (learned schedule) the MLP.
@@ -915,6 +917,10 @@ __global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
         m->dec_heur_mode = hmode;
         for (int i = 0; i < kIn; i++) m->feat[i] = s_f[i];
         for (int i = 0; i < kOut; i++) m->logits[i] = by_mlp ? s_o[i] : 0.f;
+        // the host polls dec_seq instead of waiting for the stream: every
+        // field above is visible to it before the sequence number
+        __threadfence_system();
+        __hip_atomic_store(&m->dec_seq, d.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1542,6 +1548,27 @@ void solve_plain(sssp_graph *g, sssp_stats *st)
     st->edges_relaxed = (int64_t)(done_at + 1) * E;
 }
 
+// Waits until the decision kernel has written round `seq` into the host
+// mirror: a spin on the mirror's sequence number (coherent host memory),
+// which sees the write about a microsecond after it lands -- a stream
+// synchronize wakes the host up later, and the GPU idles until the round's
+// kernels are launched. After kSpinPolls polls it falls back to the stream
+// synchronize, which also reports a failed kernel.
+constexpr int64_t kSpinPolls = (int64_t)1 << 22;
+
+void wait_decision(sssp_graph *g, int32_t seq)
+{
+    volatile int32_t *p = &g->ctrl_host->dec_seq;
+    for (int64_t i = 0; g->decide_spin && i < kSpinPolls; i++) {
+        if (*p == seq) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            return;
+        }
+        if ((i & 1023) == 1023 && hipStreamQuery(g->stream) != hipErrorNotReady) break;
+    }
+    SSSP_HIP(hipStreamSynchronize(g->stream));
+}
+
 // One round r: round-begin kernel, decision, host read-back of the frontier
 // counters, then the round itself. Returns false, having run nothing, when
 // the solve is over (no frontier, no pile, fixpoint check done or off).
@@ -1572,9 +1599,10 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     da.mlp = (learned && net) ? mlp_device_params(net) : nullptr;
     da.pile = g->pile_cur;
     da.mirror = g->ctrl_host_dev;
+    da.seq = ++g->dec_seq;
     k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
     SSSP_LAUNCH_CHECK("k_decide");
-    SSSP_HIP(hipStreamSynchronize(s));
+    wait_decision(g, da.seq);
     const RoundSlot cur = g->ctrl_host->slot[r & 1];
     const int32_t piled = g->ctrl_host->pile_n[g->pile_cur];
     const int64_t piled_edges = g->ctrl_host->pile_edges[g->pile_cur];

@@ -669,6 +669,47 @@ void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int w
     if (hb[1]) throw Error{SSSP_ERR_ARG, std::to_string(hb[1]) + " edges have a negative or NaN weight"};
 }
 
+// The tiles of the destination-blocked list: runs of at most `cap` edges
+// (a multiple of 4) of one block, `split` when a block has several; order 0 =
+// smallest tiles first (stable by block), 1 = by block, 2 = largest first,
+// 3 = by descending block. bo: padded block offsets, nblk + 1 values.
+std::vector<BlkTile> make_blk_tiles(const std::vector<int64_t> &bo, int64_t cap, int order)
+{
+    std::vector<BlkTile> tiles;
+    const int32_t nblk = (int32_t)bo.size() - 1;
+    for (int32_t b = 0; b < nblk; b++) {
+        const int64_t c = bo[(size_t)b + 1] - bo[(size_t)b];
+        const int64_t nt = (c + cap - 1) / cap;
+        for (int64_t k = 0; k < nt; k++) {
+            const int64_t e0 = bo[(size_t)b] + k * cap;
+            tiles.push_back(BlkTile{b, nt > 1 ? 1 : 0, e0, std::min(e0 + cap, bo[(size_t)b + 1])});
+        }
+    }
+    if (order == 0 || order == 2)
+        std::stable_sort(tiles.begin(), tiles.end(), [order](const BlkTile &a, const BlkTile &b) {
+            return order == 0 ? a.e1 - a.e0 < b.e1 - b.e0 : a.e1 - a.e0 > b.e1 - b.e0;
+        });
+    else if (order == 3)
+        std::stable_sort(tiles.begin(), tiles.end(),
+                         [](const BlkTile &a, const BlkTile &b) { return a.blk > b.blk; });
+    return tiles;
+}
+
+void retile_blocked(sssp_graph *g, int64_t cap, int order)
+{
+    if (g->b_off_host.empty()) return;
+    std::vector<BlkTile> tiles = make_blk_tiles(g->b_off_host, cap, order);
+    BlkTile *td = dmalloc<BlkTile>(std::max<size_t>(tiles.size(), 1));
+    SSSP_HIP(hipMemcpyAsync(td, tiles.data(), sizeof(BlkTile) * tiles.size(), hipMemcpyHostToDevice,
+                            g->stream));
+    SSSP_HIP(hipStreamSynchronize(g->stream));
+    dfree(g->b_tile);
+    g->b_tile = td;
+    g->n_btile = (int32_t)tiles.size();
+    g->blk_tile_cap = cap;
+    g->blk_tile_order = order;
+}
+
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
                    bool build_csc)
 {
@@ -860,30 +901,21 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         for (int32_t b = 0; b < nblk; b++)
             max_blk = std::max(max_blk, bp[(size_t)b + 1] - bp[(size_t)b]);
         if ((double)max_blk * nblk < 2.0 * (double)E) g->dense_all_frac = 0.4;
-        // Tile cap: 2^17 edges (fastest at 2^27-2^29 edges), doubled on larger
-        // graphs until about kBlkTileCount tiles cover them -- the 2.08 G-edge
-        // R-MAT graph ran 50.9 GTEPS with 2^19-edge tiles, 46.7 with 2^17.
-        // A power of two, so every tile starts on a 4-edge group.
+        for (int32_t b = 0; b < nblk; b++)
+            bo[(size_t)b + 1] = bo[(size_t)b] + (bp[(size_t)b + 1] - bp[(size_t)b] + 3) / 4 * 4;
+        // Tile cap: 2^17 edges below 2^29 edges, doubled while kBlkTileCount
+        // tiles of the cap would not exceed the graph -- the 2.08 G-edge R-MAT
+        // graph ran 50.9 GTEPS with 2^19-edge tiles, 46.7 with 2^17; with
+        // degree-ordered slots R-MAT 24 x 16 ran 11.3 / 11.4 ms (learned) at
+        // 2^17 / 2^18 and R-MAT 25 x 16 (2^29 edges) 19.6 / 19.2 ms, its
+        // dense round 1.58 / 1.43 ms. A power of two, so every tile starts on
+        // a 4-edge group.
         int64_t tcap = kBlkTileEdges;
-        while (tcap * kBlkTileCount < E) tcap <<= 1;
-        std::vector<BlkTile> tiles;
-        for (int32_t b = 0; b < nblk; b++) {
-            const int64_t c = (bp[(size_t)b + 1] - bp[(size_t)b] + 3) / 4 * 4;
-            bo[(size_t)b + 1] = bo[(size_t)b] + c;
-            const int64_t nt = (c + tcap - 1) / tcap;
-            for (int64_t k = 0; k < nt; k++) {
-                const int64_t e0 = bo[(size_t)b] + k * tcap;
-                tiles.push_back(BlkTile{b, nt > 1 ? 1 : 0, e0,
-                                        std::min(e0 + tcap, bo[(size_t)b + 1])});
-            }
-        }
-        // smallest tiles first (the dispatch order): the high in-degree
-        // blocks then run last, reading the distances the rest of the sweep
-        // has already lowered; measured against largest first, descending
-        // and ascending block id, it needs the fewest and cheapest rounds
-        std::stable_sort(tiles.begin(), tiles.end(), [](const BlkTile &a, const BlkTile &b) {
-            return a.e1 - a.e0 < b.e1 - b.e0;
-        });
+        while (tcap * kBlkTileCount <= E) tcap <<= 1;
+        g->b_off_host = bo;
+        std::vector<BlkTile> tiles = make_blk_tiles(bo, tcap, 0);
+        g->blk_tile_cap = tcap;
+        g->blk_tile_order = 0;
         const int64_t Ep = bo[(size_t)nblk];
         SSSP_HIP(hipMemcpyAsync(boff, bo.data(), 8 * bo.size(), hipMemcpyHostToDevice, s));
         g->b_src = dmalloc<int32_t>((size_t)Ep);

@@ -55,14 +55,16 @@ struct Ctrl {
     uint32_t pile_min[2];     // min distance over the pile when it was built
     int64_t pile_edges[2];    // out-edges of the pile's vertices
     int32_t lb_n;             // queue entries plus pile entries moved onto the queue
+    int32_t dec_seq;          // host mirror: written last by the decision kernel (round
+                              // sequence number), after the fields above
 };
 
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
 // block's run padded to a multiple of 4 edges with weight-0 self loops on the
 // block's first vertex; destinations are stored as 16-bit offsets in their
 // block. A tile is a run of at most kBlkTileEdges edges of one block (the
-// cap doubles until the graph has at most kBlkTileCount tiles' worth of
-// edges); `split` = the block has more than one tile.
+// cap doubles while the graph holds kBlkTileCount tiles' worth of edges or
+// more); `split` = the block has more than one tile.
 constexpr int kBlkBits = 14;
 static_assert(kBlkBits <= 16, "block offsets are stored in 16 bits");
 constexpr int64_t kBlkTileEdges = (int64_t)1 << 17;
@@ -129,6 +131,9 @@ struct sssp_graph {
     uint32_t *b_w = nullptr;
     sssp::BlkTile *b_tile = nullptr;
     int32_t n_btile = 0;
+    std::vector<int64_t> b_off_host;  // padded block offsets (tiles are rebuilt from them)
+    int64_t blk_tile_cap = 0;         // edges per tile at most
+    int blk_tile_order = 0;           // dispatch order of the tiles (make_blk_tiles)
     int64_t b_edges = 0;  // edges in the blocked list, padding included
     // multi-source batches (built on first use): records, tiles, and the
     // vertex-major distances of up to 256 sources (D[v * n + s])
@@ -202,6 +207,8 @@ struct sssp_graph {
     sssp::Ctrl *ctrl_host = nullptr;  // pinned mirror (coherent, mapped)
     sssp::Ctrl *ctrl_host_dev = nullptr;  // the mirror's device address
     int32_t tick = 1;
+    int32_t dec_seq = 0;  // sequence number of the last decision enqueued
+    int decide_spin = 1;  // 1: poll the mirror for the decision; 0: stream synchronize
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_r0 = nullptr, ev_r1 = nullptr;
 
     // knobs
@@ -309,6 +316,10 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
                    bool build_csc);
 // the multi-source records and tiles (m_rec, m_tile), built once per graph
 void build_multi_layout(sssp_graph *g);
+// rebuilds the destination-blocked list's tile table (cap: edges per tile, a
+// multiple of 4; order: 0 smallest first, 1 by block, 2 largest first, 3 by
+// descending block)
+void retile_blocked(sssp_graph *g, int64_t cap, int order);
 void device_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *blk_a,
                                int64_t *blk_b, int64_t n_blk_cap, hipStream_t s);
 int64_t scan_blocks_needed(int64_t n);

@@ -300,6 +300,30 @@ def test_plain_solve_queued_and_traced(gpu, oracle):
 
 
 @pytest.mark.gpu
+def test_blocked_tiles_any_cap_and_order(gpu, oracle):
+    """The dense rounds' tile table rebuilt with other caps (edges per tile,
+    down to 256: most blocks split over several workgroups) and dispatch
+    orders: every dense round still ends on the oracle's fixpoint."""
+    src, dst, w = S.rmat(14, 16, seed=43, threads=16)
+    V = 1 << 14
+    ref = oracle.dijkstra(src, dst, w, V, 3)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)
+        g.set_param("dense_all_frac", 0.0)  # every all-edge round dense
+        for cap, order in ((256, 0), (1024, 1), (4096, 2), (1 << 20, 3), (1 << 17, 0)):
+            print(f"blocked tiles: cap {cap}, order {order}", flush=True)
+            g.set_param("blk_tile_cap", cap)
+            g.set_param("blk_tile_order", order)
+            assert g.get_param("blk_tile_cap") == cap and g.get_param("blk_tile_order") == order
+            for sched in ("all", "heuristic"):
+                d, _ = g.solve(3, sched)
+                assert np.array_equal(bits(d), bits(ref)), (cap, order, sched)
+        for name, bad in (("blk_tile_cap", 1002), ("blk_tile_cap", 128), ("blk_tile_order", 4)):
+            with pytest.raises(S.SSSPError):
+                g.set_param(name, bad)
+
+
+@pytest.mark.gpu
 def test_batch_u32(gpu, oracle):
     src, dst, w = S.er(1 << 12, 8, seed=4, weight_mode=1)
     V = 1 << 12
