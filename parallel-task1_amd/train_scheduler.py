@@ -18,6 +18,14 @@ models/scheduler_mlp.f32 (raw float32, the library's parameter order) with a
 JSON sidecar describing the run.
 
     python parallel-task1_amd/train_scheduler.py [--scales 19 20 21 22 23 24]
+
+The committed network (models/scheduler_mlp.f32) was trained on the rows of
+two collections over the default graphs -- one whose rollouts continued with
+the previous network (--rollout-weights), one whose rollouts followed the
+rule -- concatenated:
+    python parallel-task1_amd/train_scheduler.py --rollout-weights prev.f32 --out A.f32
+    python parallel-task1_amd/train_scheduler.py --out B.f32
+    python parallel-task1_amd/train_scheduler.py --data A_data.npz B_data.npz
 """
 import argparse
 import json
@@ -143,7 +151,8 @@ def main():
                     help="rounds of SSSP solves or of BFS (unit weights)")
     ap.add_argument("--out", help="default: models/scheduler_mlp.f32 (sssp), "
                                   "models/scheduler_bfs_mlp.f32 (bfs)")
-    ap.add_argument("--data", help="train on the X/T arrays of an earlier run's _data.npz")
+    ap.add_argument("--data", nargs="+",
+                    help="train on the X/T arrays of earlier runs' _data.npz files (concatenated)")
     ap.add_argument("--rollout-weights",
                     help="weights of a network the rollouts continue with (policy iteration); "
                          "default: the hand-written rule")
@@ -157,9 +166,13 @@ def main():
                                 else "scheduler_bfs_mlp.f32")
 
     if args.data:
-        with np.load(args.data) as z:
-            X, T = z["X"], z["T"]
-        meta = [{"data": os.path.basename(args.data)}]
+        Xs, Ts = [], []
+        for path in args.data:
+            with np.load(path) as z:
+                Xs.append(z["X"])
+                Ts.append(z["T"])
+        X, T = np.concatenate(Xs), np.concatenate(Ts)
+        meta = [{"data": os.path.basename(p)} for p in args.data]
     else:
         net = None
         if args.rollout_weights:

@@ -97,32 +97,40 @@ def test_any_network_keeps_the_fixpoint(gpu, oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.timeout(600)
 def test_trained_network_against_heuristic(gpu):
-    """Reports what the trained network gains over the hand-written rule on an
-    R-MAT graph of scale 21 (inside the training range but unseen seed), and
-    checks that it does not lose much; distances must be identical."""
+    """What the trained network gains over the hand-written rule on an R-MAT
+    graph no training run saw (scale 24, edge factor 16, seed 99; training
+    used seeds 7-12 and 57-60 and uniform graphs): 8 sources, the two
+    schedules interleaved, best of three timings each. The learned schedule
+    must not be slower than the rule, and its distances are identical."""
     if not os.path.exists(WEIGHTS):
         pytest.skip("no trained weights (run parallel-task1_amd/train_scheduler.py)")
-    g = S.DeviceGraph.rmat(21, 16, seed=99, threads=64)
+    g = S.DeviceGraph.rmat(24, 16, seed=99, threads=64)
     try:
         deg = g.out_degrees()
-        srcs = S.pick_sources(np.repeat(np.arange(g.V, dtype=np.int32), deg), g.V, 99, 4)
+        srcs = S.pick_sources(np.nonzero(deg)[0].astype(np.int32), g.V, 99, 8)
         m = S.MLP(0)
         m.set_params(np.fromfile(WEIGHTS, np.float32))
         g.set_mlp(m)
         th, tl = [], []
         for s in srcs:
-            for _ in range(2):  # warm
-                dh, sh = g.solve(int(s), "heuristic")
-                dl, sl = g.solve(int(s), "learned")
+            dh, _ = g.solve(int(s), "heuristic")
+            dl, sl = g.solve(int(s), "learned")
             assert np.array_equal(bits(dh), bits(dl))
             assert sl["mlp_rounds"] > 0
-            th.append(sh["t_solve_ms"])
-            tl.append(sl["t_solve_ms"])
+            bh = bl = float("inf")
+            for _ in range(3):
+                _, sh = g.solve(int(s), "heuristic", download=False)
+                _, sl = g.solve(int(s), "learned", download=False)
+                bh = min(bh, sh["t_solve_ms"])
+                bl = min(bl, sl["t_solve_ms"])
+            th.append(bh)
+            tl.append(bl)
         gain = sum(th) / sum(tl)
-        print(f"learned vs heuristic on RMAT-21: {gain:.3f}x (heuristic {sum(th):.2f} ms, "
+        print(f"learned vs heuristic on held-out RMAT-24: {gain:.3f}x (heuristic {sum(th):.2f} ms, "
               f"learned {sum(tl):.2f} ms over {len(srcs)} sources)")
-        assert gain > 0.8
+        assert gain >= 1.0, f"learned schedule {gain:.3f}x the rule"
     finally:
         g.close()
 
