@@ -66,6 +66,8 @@ extern "C" {
 #define SSSP_INPUT_DEVICE 1u   /* src/dst/w are device pointers on `device` */
 #define SSSP_NO_CSC       2u   /* skip the CSC (pull) layout */
 #define SSSP_NO_RELABEL   4u   /* keep vertex ids as the layouts' order (no degree-ordered slots) */
+#define SSSP_RELABEL      8u   /* degree-ordered slots whatever the graph (default: from 2^23
+                                  vertices on skewed graphs, see sssp_graph_vertex_order) */
 
 typedef struct sssp_graph sssp_graph;
 typedef struct sssp_pack  sssp_pack;
@@ -141,11 +143,13 @@ int sssp_graph_destroy(sssp_graph *g);
 int sssp_graph_info(const sssp_graph *g, int32_t *vertices, int64_t *edges,
                     int32_t *weight_type, int64_t *device_bytes);
 /* The order the device layouts use: slot_of_vertex[v] (host, V elements) =
- * the slot of vertex v. Unless SSSP_NO_RELABEL was given at creation, slots
- * rank the vertices by descending in- plus out-degree (ties by id), so that
- * the distances most edges read share cache lines; every argument and result
- * of this API stays in the caller's vertex ids. No counterpart in the
- * reference (its layouts keep the ids). */
+ * the slot of vertex v. On a graph of at least 2^23 vertices whose largest
+ * in- plus out-degree is at least 16 times the mean (R-MAT-like), or with
+ * SSSP_RELABEL, slots rank the vertices by descending in- plus out-degree
+ * (ties by descending out-degree, then id), so that the distances most edges
+ * read share cache lines; otherwise (or with SSSP_NO_RELABEL) slot = vertex.
+ * Every argument and result of this API stays in the caller's vertex ids. No
+ * counterpart in the reference (its layouts keep the ids). */
 int sssp_graph_vertex_order(const sssp_graph *g, int32_t *slot_of_vertex);
 /* Copies the device layouts back (tests): row_ptr[V+1], sorted src/dst/w[E];
  * col_ptr[V+1], csc_src[E], csc_w[E] (csc arrays may be NULL). The layouts

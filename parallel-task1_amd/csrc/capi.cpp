@@ -170,7 +170,9 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
                   uint32_t flags)
 {
     hipStream_t s = g->stream;
-    g->relabel = !(flags & SSSP_NO_RELABEL);
+    SSSP_REQUIRE(!((flags & SSSP_NO_RELABEL) && (flags & SSSP_RELABEL)),
+                 "SSSP_RELABEL and SSSP_NO_RELABEL exclude each other");
+    g->relabel_mode = (flags & SSSP_RELABEL) ? 1 : (flags & SSSP_NO_RELABEL) ? -1 : 0;
     build_layouts(g, d_src, d_dst, d_w, !(flags & SSSP_NO_CSC));
     const size_t V = (size_t)g->V;
     g->dist = dmalloc<uint32_t>(V);

@@ -94,6 +94,16 @@ __global__ __launch_bounds__(kBlock) void k_degrees(const int32_t *src, const in
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_max_u32(const uint32_t *x, int64_t n, uint32_t *mx)
+{
+    uint32_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        m = max(m, x[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o));
+    if (__lane_id() == 0 && m) atomicMax(mx, m);
+}
+
 // ascending sort key = descending count (counts above kOrderCap tie), of
 // vertex at[i] (or of vertex i when `at` is null)
 constexpr uint32_t kOrderCap = 1u << 30;
@@ -745,19 +755,45 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
     // with few one bits, scattered over the whole id range, so the distance
     // gathers of a dense round touch a cache line per hub; in slot order the
     // hubs' distances are one dense prefix, and among the many vertices of
-    // equal small degree the more often gathered come first. Measured at
-    // R-MAT scale 25 (dense round per launch, learned solve): ids 2.04 /
+    // equal small degree the more often gathered come first. Orders measured
+    // at R-MAT scale 25 (dense round per launch, learned solve): ids 2.04 /
     // 21.3 ms; total degree, ties by id 1.70 / 20.0; out-degree 1.79 / 20.5;
     // in-degree 1.68 / 20.1; total degree, ties by out-degree 1.61 / 19.0.
-    if (g->relabel && E > 0 && V > 1) {
-        uint32_t *tot = dmalloc<uint32_t>((size_t)V);
-        scratch.p[6] = tot;
-        uint32_t *outd = dmalloc<uint32_t>((size_t)V);
-        scratch.p[7] = outd;
-        SSSP_HIP(hipMemsetAsync(tot, 0, 4 * (size_t)V, s));
-        SSSP_HIP(hipMemsetAsync(outd, 0, 4 * (size_t)V, s));
-        k_degrees<<<grid_for(E), kBlock, 0, s>>>(d_src, d_dst, E, tot, outd);
-        SSSP_LAUNCH_CHECK("k_degrees");
+    // It pays from 2^23 vertices on skewed graphs (learned R-MAT GTEPS with /
+    // without: scale 23 21.4 / 19.0, 24 24.5 / 22.0, 25 +12 %), not below or
+    // without hubs (scale 22 x 16 14.6 / 14.6, 22 x 32 20.3 / 22.7, uniform
+    // 2^24 x 16 7.9 / 8.2): by default the layouts use it when V >= 2^23 and
+    // the largest total degree is at least 16 times the mean.
+    uint32_t *tot = nullptr, *outd = nullptr;
+    if (g->relabel_mode > 0 || (g->relabel_mode == 0 && V >= (1 << 23))) {
+        if (E > 0 && V > 1) {
+            tot = dmalloc<uint32_t>((size_t)V);
+            scratch.p[6] = tot;
+            outd = dmalloc<uint32_t>((size_t)V);
+            scratch.p[7] = outd;
+            SSSP_HIP(hipMemsetAsync(tot, 0, 4 * (size_t)V, s));
+            SSSP_HIP(hipMemsetAsync(outd, 0, 4 * (size_t)V, s));
+            k_degrees<<<grid_for(E), kBlock, 0, s>>>(d_src, d_dst, E, tot, outd);
+            SSSP_LAUNCH_CHECK("k_degrees");
+        }
+        g->relabel = tot != nullptr;
+        if (g->relabel && g->relabel_mode == 0) {
+            uint32_t *mx = dmalloc<uint32_t>(1);
+            struct F1 {
+                void *p;
+                ~F1() { dfree(p); }
+            } f1{mx};
+            SSSP_HIP(hipMemsetAsync(mx, 0, 4, s));
+            k_max_u32<<<grid_for(V), kBlock, 0, s>>>(tot, V, mx);
+            SSSP_LAUNCH_CHECK("k_max_u32");
+            uint32_t hmax = 0;
+            SSSP_HIP(hipMemcpyAsync(&hmax, mx, 4, hipMemcpyDeviceToHost, s));
+            SSSP_HIP(hipStreamSynchronize(s));
+            // mean total degree = 2E / V
+            g->relabel = (double)hmax * (double)V >= 16.0 * 2.0 * (double)E;
+        }
+    }
+    if (g->relabel) {
         uint32_t *okeys = dmalloc<uint32_t>((size_t)V);
         uint32_t *by_out = dmalloc<uint32_t>((size_t)V);
         struct F0 {
@@ -800,7 +836,9 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         d_src = rs;
         d_dst = rd;
     } else {
-        g->relabel = false;
+        dfree(tot);
+        dfree(outd);
+        scratch.p[6] = scratch.p[7] = nullptr;
     }
 
     // ---- CSR / src-sorted edge list ----

@@ -177,7 +177,8 @@ struct sssp_graph {
     // id), so that the distances of the vertices most edges read share cache
     // lines. vperm[v] = slot of caller vertex v, vinv[slot] = caller vertex
     // (both null, and slot = vertex, when the graph is not relabelled)
-    bool relabel = false;
+    int relabel_mode = 0;  // 0: by size and skew, 1: always, -1: never
+    bool relabel = false;  // the layouts use degree-ordered slots
     int32_t *vperm = nullptr;
     int32_t *vinv = nullptr;
     std::vector<int32_t> vperm_host;

@@ -28,7 +28,7 @@ OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_INTERNAL, ERR_IO, ERR_NOT_CONVERGED = range(7
 W_F32, W_U32, W_F64 = 0, 1, 2
 SCHED_PLAIN, SCHED_HEURISTIC, SCHED_LEARNED, SCHED_PUSH, SCHED_ALL, SCHED_PULL = range(6)
 MODE_PUSH, MODE_ALL, MODE_PULL, MODE_VERIFY, MODE_DENSE = 0, 1, 2, 3, 4
-INPUT_DEVICE, NO_CSC, NO_RELABEL = 1, 2, 4
+INPUT_DEVICE, NO_CSC, NO_RELABEL, RELABEL = 1, 2, 4, 8
 MLP_IN, MLP_HID, MLP_MODES, MLP_CHUNKS = 5, 64, 3, 4
 MLP_OUT = MLP_MODES + MLP_CHUNKS
 MLP_NPARAM = MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_OUT + MLP_OUT
@@ -239,7 +239,10 @@ def features(fv, fe, unsettled, rnd, mean_deg, V, E):
 # device graph
 # --------------------------------------------------------------------------
 def _flags(csc, relabel):
-    return (0 if csc else NO_CSC) | (0 if relabel else NO_RELABEL)
+    """relabel: None = the library's choice (degree-ordered slots on skewed
+    graphs from 2^23 vertices), True = always, False = never."""
+    order = 0 if relabel is None else (RELABEL if relabel else NO_RELABEL)
+    return (0 if csc else NO_CSC) | order
 
 
 class DeviceGraph:
@@ -247,7 +250,7 @@ class DeviceGraph:
 
     @classmethod
     def rmat(cls, scale, edge_factor, a=57, b=19, c=19, d=5, seed=1, threads=64, device=0,
-             csc=True, download_edges=False, relabel=True):
+             csc=True, download_edges=False, relabel=None):
         """Generates the R-MAT graph on the device (bit-identical to rmat())
         and builds it there; optionally returns the generated edge list."""
         self = cls.__new__(cls)
@@ -267,7 +270,7 @@ class DeviceGraph:
 
     @classmethod
     def from_bin(cls, path, directed=True, weight_dtype=np.float32, device=0, csc=True,
-                 relabel=True):
+                 relabel=None):
         """Device ingest of a binary edge-list file (the reference's format):
         chunks stream through pinned buffers onto the device, where the
         layouts are built; no host copy of the graph is made."""
@@ -290,7 +293,7 @@ class DeviceGraph:
         return self
 
     def __init__(self, src, dst, w, vertices: int, device: int = 0, csc: bool = True,
-                 relabel: bool = True):
+                 relabel=None):
         w = np.ascontiguousarray(w)
         if w.dtype == np.float32:
             self.wtype, self.wdtype = W_F32, np.float32
@@ -382,7 +385,7 @@ class DeviceGraph:
 
     def vertex_order(self):
         """slot[v]: the position of vertex v in the device layouts (the
-        identity when the graph was built with relabel=False)."""
+        identity when the graph keeps its vertex ids)."""
         o = np.empty(self.V, np.int32)
         _check(lib().sssp_graph_vertex_order(self._h, _ptr(o)))
         return o

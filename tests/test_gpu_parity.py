@@ -119,12 +119,13 @@ def test_layout_is_stable_sort(gpu, relabel):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("relabel", [None, True], ids=["ids", "slots"])
 @pytest.mark.parametrize("scale,ef,seed,threads", [(10, 16, 1, 64), (14, 16, 9, 64), (16, 16, 0, 1)])
-def test_rmat_every_schedule_bit_exact(gpu, oracle, scale, ef, seed, threads):
+def test_rmat_every_schedule_bit_exact(gpu, oracle, scale, ef, seed, threads, relabel):
     src, dst, w = S.rmat(scale, ef, seed=seed, threads=threads)
     V = 1 << scale
     sources = [0] + list(S.pick_sources(src, V, seed + 100, 3))
-    with S.DeviceGraph(src, dst, w, V) as g:
+    with S.DeviceGraph(src, dst, w, V, relabel=relabel) as g:
         g.set_param("plain_below_edges", 0)  # exercise the round engine
         for s in sources:
             ref, _ = oracle.bf(src, dst, w, V, s)
@@ -226,7 +227,7 @@ def test_batch_sources_match_single_solves(gpu, oracle, policy):
     src, dst, w = S.rmat(13, 16, seed=21, threads=16)
     V = 1 << 13
     sources = S.pick_sources(src, V, 21, 48)
-    with S.DeviceGraph(src, dst, w, V) as g:
+    with S.DeviceGraph(src, dst, w, V, relabel=True) as g:
         g.set_param("plain_below_edges", 0)  # exercise the round engine
         D, st = g.solve_batch(sources, policy=policy)
         assert D.shape == (48, V) and st["t_solve_ms"] > 0
@@ -251,7 +252,7 @@ def test_batch_captured_launch_replays(gpu, oracle):
     queued from the host and to policy 1 (a workgroup per source)."""
     src, dst, w = S.rmat(14, 16, seed=41, threads=16)
     V = 1 << 14
-    with S.DeviceGraph(src, dst, w, V) as g:
+    with S.DeviceGraph(src, dst, w, V, relabel=True) as g:
         for k in range(3):
             sources = S.pick_sources(src, V, 41 + k, 64)
             print(f"captured batch replay {k}: 64 sources, RMAT-14", flush=True)
@@ -425,7 +426,7 @@ def test_far_pile_keeps_the_fixpoint(gpu, oracle, delta, hub):
     far pile; mixed schedules flush it before all-edge and pull rounds."""
     src, dst, w = S.rmat(14, 16, seed=31, threads=16)
     V = 1 << 14
-    with S.DeviceGraph(src, dst, w, V) as g:
+    with S.DeviceGraph(src, dst, w, V, relabel=True) as g:
         g.set_param("plain_below_edges", 0)  # exercise the round engine
         g.set_param("delta", delta)
         g.set_param("hub_degree", hub)
@@ -516,7 +517,7 @@ def test_bfs_small_graphs_every_schedule(gpu, oracle, name, V, src, dst, w):
 def test_bfs_rmat_and_er_every_schedule(gpu, oracle):
     src, dst, w = S.rmat(14, 16, seed=17, threads=16)
     V = 1 << 14
-    with S.DeviceGraph(src, dst, w, V) as g:
+    with S.DeviceGraph(src, dst, w, V, relabel=True) as g:
         g.set_param("plain_below_edges", 0)  # exercise the round engine
         for s in [0] + list(S.pick_sources(src, V, 5, 3)):
             vis_ref, lev_ref, _ = oracle.bfs(src, dst, V, int(s))
@@ -639,7 +640,7 @@ def test_dense_rounds_with_split_blocks(gpu, oracle):
     per_block = np.bincount(dst >> 14)
     assert per_block.max() > (1 << 19), per_block.max()  # at least one split block
     wu = (np.arange(len(src)) % 255 + 1).astype(np.uint32)
-    with S.DeviceGraph(src, dst, w, V) as g, S.DeviceGraph(src, dst, wu, V) as gu:
+    with S.DeviceGraph(src, dst, w, V, relabel=True) as g, S.DeviceGraph(src, dst, wu, V) as gu:
         for h in (g, gu):
             h.set_param("plain_below_edges", 0)
             h.set_param("dense_all_frac", 0.0)  # every all-edge round dense
