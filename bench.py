@@ -11,19 +11,22 @@ steps * E / wall time of the timed steps
 
 Every source is solved with the learned, heuristic and plain schedules outside
 the timed region; the three distance arrays must be bit-identical and the
-learned one must pass the GPU certificate. Besides the three schedules, the
-line reports all_dense_gteps: every round an all-edge round run dense (plain
-Bellman-Ford over the destination-blocked edge list), which separates the
-dense round kernel's share of the learned/plain ratio from the schedule's.
+learned one must pass the GPU certificate. The plain schedule is plain
+Bellman-Ford with the library's default pass kernel for the graph (at this
+size the destination-blocked pass, k_blk_plain); besides it the line reports
+plain_edge_list_gteps (the same passes over the XCD-partitioned edge list,
+k_el_pass) and all_dense_gteps (the round engine with every round an
+all-edge round run dense).
 
 roofline: the learned schedule's dominant kernel, the dense all-edge round
 (k_blk_pass), timed live with HIP events around each of its launches in one
-learned solve; plain_pass_roofline: the plain all-edge pass (k_el_pass) over
-one plain solve. HBM traffic per launch (traffic): before this process
-touches the GPU it runs itself twice under `rocprofv3 --pmc` (FETCH_SIZE, then
-WRITE_SIZE) in --pmc-child mode, which copies 1 GiB device to device and runs
-one plain and one learned solve; each counter is scaled by its value for the
-copy. --traffic off skips this (traffic = null).
+learned solve; plain_pass_roofline: the plain pass over one plain solve;
+edge_list_pass_roofline: k_el_pass over one plain solve on the edge list.
+HBM traffic per launch (traffic): before this process touches the GPU it runs
+itself twice under `rocprofv3 --pmc` (FETCH_SIZE, then WRITE_SIZE) in
+--pmc-child mode, which copies 1 GiB device to device and runs one plain solve
+with each pass kernel and one learned solve; each counter is scaled by its
+value for the copy. --traffic off skips this (traffic = null).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--traffic auto|off]
 """
@@ -134,14 +137,21 @@ def cpu_baseline(src, dst, w, V, source, plain_passes, max_passes):
 
 
 COPY_BYTES = 1 << 30
-PASS_KERNEL = "k_el_pass"
+EDGE_LIST_PASS = "k_el_pass"
+BLOCKED_PASS = "k_blk_plain"
 DENSE_KERNEL = "k_blk_pass"
+
+
+def pass_kernel(g):
+    """Name of the kernel the plain schedule runs on graph g."""
+    return BLOCKED_PASS if g.get_param("plain_blocked") else EDGE_LIST_PASS
 
 
 def pmc_child(args):
     """Profiled child: a 1 GiB device-to-device copy (calibration), then one
-    plain solve and one learned solve of the bench graph from the first bench
-    source; prints the plain solve's pass count and the learned solve's dense
+    plain solve with the default pass kernel, one with the edge-list pass and
+    one learned solve of the bench graph from the first bench source; prints
+    the pass counts, the default pass kernel and the learned solve's dense
     rounds (k_blk_pass launches)."""
     hip = C.CDLL("libamdhip64.so")
     a, b = C.c_void_p(), C.c_void_p()
@@ -156,14 +166,21 @@ def pmc_child(args):
     hip.hipFree(b)
     g = S.DeviceGraph.rmat(args.scale, args.edge_factor, seed=args.seed, threads=64)
     source = bench_sources(g, args)[0]
+    kern = pass_kernel(g)
     _, st = g.solve(int(source), "plain", download=False)
+    passes = {kern: int(st["iterations"])}
+    if kern != EDGE_LIST_PASS:
+        g.set_param("plain_kernel", 1)
+        _, st = g.solve(int(source), "plain", download=False)
+        g.set_param("plain_kernel", 0)
+        passes[EDGE_LIST_PASS] = int(st["iterations"])
     if os.path.exists(WEIGHTS):
         mlp = S.MLP(0)
         mlp.set_params(np.fromfile(WEIGHTS, np.float32))
         g.set_mlp(mlp)
     g.solve(int(source), "learned", download=False)
     dense = int(np.isin(g.trace()["mode"], (S.MODE_VERIFY, S.MODE_DENSE)).sum())
-    print(json.dumps({"passes": int(st["iterations"]), "dense_rounds": dense,
+    print(json.dumps({"passes": passes, "pass_kernel": kern, "dense_rounds": dense,
                       "source": int(source)}), flush=True)
 
 
@@ -200,7 +217,8 @@ def _pmc_run(counter, args, workdir):
     if child is None or not files:
         raise RuntimeError(f"rocprofv3 --pmc {counter}: no output")
     passes, dense = child["passes"], child["dense_rounds"]
-    copy_kib, pass_kib, dense_kib = [], [], []
+    copy_kib, dense_kib = [], []
+    pass_kib = {k: [] for k in passes}
     with open(files[0]) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] != counter:
@@ -208,17 +226,20 @@ def _pmc_run(counter, args, workdir):
             name, v = r["Kernel_Name"], float(r["Counter_Value"])
             if "copyBuffer" in name:
                 copy_kib.append(v)
-            elif PASS_KERNEL in name:
-                pass_kib.append(v)
             elif DENSE_KERNEL in name:
                 dense_kib.append(v)
-    if not copy_kib or len(pass_kib) < passes or len(dense_kib) != dense:
+            else:
+                for k in pass_kib:
+                    if k in name:
+                        pass_kib[k].append(v)
+    if (not copy_kib or len(dense_kib) != dense
+            or any(len(pass_kib[k]) < n for k, n in passes.items())):
         raise RuntimeError(f"rocprofv3 --pmc {counter}: dispatches missing")
-    # the copy's counter in KiB against the bytes it moves; the plain solve's
+    # the copy's counter in KiB against the bytes it moves; a plain solve's
     # first `passes` pass dispatches are its real passes (later queued ones
     # return at once); every dense-kernel dispatch is a dense round
     factor = COPY_BYTES / (max(copy_kib) * 1024.0)
-    per_pass = float(np.mean(pass_kib[:passes])) * 1024.0 * factor
+    per_pass = {k: float(np.mean(v[:passes[k]])) * 1024.0 * factor for k, v in pass_kib.items()}
     per_dense = float(np.mean(dense_kib)) * 1024.0 * factor if dense else None
     return per_pass, per_dense, factor, passes, dense
 
@@ -253,13 +274,13 @@ def copy_bandwidth(reps=5):
 
 
 def measure_traffic(args):
-    """HBM bytes per launch (FETCH + WRITE, copy-calibrated) of the plain
-    all-edge pass and of the dense round kernel."""
+    """HBM bytes per launch (FETCH + WRITE, copy-calibrated) of each plain
+    pass kernel and of the dense round kernel."""
     with tempfile.TemporaryDirectory(dir="/tmp") as td:
         fetch, dfetch, ffac, passes, dense = _pmc_run("FETCH_SIZE", args, td)
         write, dwrite, wfac, _, _ = _pmc_run("WRITE_SIZE", args, td)
-    out = {"plain_pass": {"bytes_per_launch": fetch + write, "fetch_bytes": fetch,
-                          "write_bytes": write, "launches": passes},
+    out = {"passes": {k: {"bytes_per_launch": fetch[k] + write[k], "fetch_bytes": fetch[k],
+                          "write_bytes": write[k], "launches": passes[k]} for k in passes},
            "dense_round": None, "fetch_factor": ffac, "write_factor": wfac}
     if dfetch is not None and dwrite is not None:
         out["dense_round"] = {"bytes_per_launch": dfetch + dwrite, "fetch_bytes": dfetch,
@@ -335,16 +356,20 @@ def main():
             certified &= bool(ok and same)
             log(f"source {int(s)}: certificate {ok} {counts.tolist()} identical {same}")
 
-    # timed runs; "all_dense" (every round an all-edge round run dense, i.e.
-    # plain Bellman-Ford over the destination-blocked list) separates what the
-    # dense round kernel gives from what the schedule gives
+    # timed runs; "plain_edge_list" (the plain passes over the XCD-partitioned
+    # edge list) and "all_dense" (the round engine, every round an all-edge
+    # round run dense) beside the three schedules
     results = {}
-    for sched in ("learned", "heuristic", "plain", "all_dense"):
+    for sched in ("learned", "heuristic", "plain", "plain_edge_list", "all_dense"):
         if sched == "all_dense":
             frac = g.get_param("dense_all_frac")
             g.set_param("dense_all_frac", 0.0)
             wall, tdev, stats = time_schedule(g, sources, "all", args.steps, args.warmup)
             g.set_param("dense_all_frac", frac)
+        elif sched == "plain_edge_list":
+            g.set_param("plain_kernel", 1)
+            wall, tdev, stats = time_schedule(g, sources, "plain", args.steps, args.warmup)
+            g.set_param("plain_kernel", 0)
         else:
             wall, tdev, stats = time_schedule(g, sources, sched, args.steps, args.warmup)
         results[sched] = {
@@ -364,26 +389,29 @@ def main():
         log(sched, json.dumps(results[sched]))
 
     # rooflines, from HIP events recorded around each round on the graph's
-    # stream: the plain all-edge pass (k_el_pass) over a plain solve of the
-    # first source, and the learned schedule's dominant kernel, the dense
-    # all-edge round (k_blk_pass: one launch per dense round and per fixpoint
-    # check, nothing else inside its events), over a learned solve of the
-    # same source. Both stream 12 B per edge and read and write d once:
+    # stream: the plain pass (the default kernel, then k_el_pass) over a plain
+    # solve of the first source, and the learned schedule's dominant kernel,
+    # the dense all-edge round (k_blk_pass: one launch per dense round and per
+    # fixpoint check, nothing else inside its events), over a learned solve of
+    # the same source. All stream the edges once and read and write d once:
     # 12 E + 8 V algorithmic bytes per launch.
     algo_bytes = 12 * E + 8 * V
+    kern = pass_kernel(g)
     g.set_param("trace_events", 1)
     _, st = g.solve(int(sources[0]), "plain", download=False)
     tr = g.trace()
+    g.set_param("plain_kernel", 1)
+    g.solve(int(sources[0]), "plain", download=False)
+    tre = g.trace()
+    g.set_param("plain_kernel", 0)
     g.solve(int(sources[0]), "learned", download=False)
     trl = g.trace()
     g.set_param("trace_events", 0)
-    pass_ms = float(np.mean(tr["ms"]))
     dense_ms_all = trl["ms"][np.isin(trl["mode"], (S.MODE_VERIFY, S.MODE_DENSE))]
     dense_ms = float(np.mean(dense_ms_all))
     copy_gbs = copy_bandwidth()
     log(f"device-to-device copy: {copy_gbs} GB/s")
     dense_t = traffic["dense_round"] if traffic else None
-    plain_t = traffic["plain_pass"] if traffic else None
     achieved = algo_bytes / (dense_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm",
                 "kernel": "k_blk_pass (dense all-edge round, destination-blocked edge list)",
@@ -394,18 +422,23 @@ def main():
                 "algorithmic_bytes_per_launch": algo_bytes,
                 "avg_launch_ms": round(dense_ms, 4), "launches": int(len(dense_ms_all)),
                 "achievable_copy_GBs": round(copy_gbs, 1) if copy_gbs else None}
-    achieved_p = algo_bytes / (pass_ms * 1e-3) / 1e9
-    plain_roofline = {"bound": "hbm",
-                      "kernel": "k_el_pass (all-edge pass, XCD-partitioned src-sorted edge list)",
-                      "achieved": round(achieved_p, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": round(achieved_p / HBM_PEAK_GBS, 4),
-                      "traffic": round(plain_t["bytes_per_launch"]) if plain_t else None,
-                      "traffic_detail": plain_t,
-                      "algorithmic_bytes_per_launch": algo_bytes,
-                      "avg_launch_ms": round(pass_ms, 4), "launches": int(len(tr["ms"]))}
     if traffic:
         roofline["calibration"] = {"fetch_factor": traffic["fetch_factor"],
                                    "write_factor": traffic["write_factor"]}
+    labels = {BLOCKED_PASS: "k_blk_plain (plain pass, destination-blocked edge list)",
+              EDGE_LIST_PASS: "k_el_pass (plain pass, XCD-partitioned src-sorted edge list)"}
+
+    def pass_roofline(name, ms):
+        pt = traffic["passes"].get(name) if traffic else None
+        a = algo_bytes / (float(np.mean(ms)) * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": labels[name], "achieved": round(a, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(a / HBM_PEAK_GBS, 4),
+                "traffic": round(pt["bytes_per_launch"]) if pt else None, "traffic_detail": pt,
+                "algorithmic_bytes_per_launch": algo_bytes,
+                "avg_launch_ms": round(float(np.mean(ms)), 4), "launches": int(len(ms))}
+
+    plain_roofline = pass_roofline(kern, tr["ms"])
+    edge_list_roofline = pass_roofline(EDGE_LIST_PASS, tre["ms"])
 
     cpu = None
     if not args.no_cpu:
@@ -432,6 +465,8 @@ def main():
                    "edges": E, "sources": [int(s) for s in sources], "schedule": "learned",
                    "parallelism": "single GPU"},
         "plain_gteps": round(results["plain"]["gteps"], 3),
+        "plain_kernel": kern,
+        "plain_edge_list_gteps": round(results["plain_edge_list"]["gteps"], 3),
         "all_dense_gteps": round(results["all_dense"]["gteps"], 3),
         "heuristic_gteps": round(results["heuristic"]["gteps"], 3),
         "speedup_vs_plain": round(learned["gteps"] / results["plain"]["gteps"], 3),
@@ -441,6 +476,7 @@ def main():
         "schedules": results,
         "roofline": roofline,
         "plain_pass_roofline": plain_roofline,
+        "edge_list_pass_roofline": edge_list_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

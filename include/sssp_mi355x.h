@@ -184,8 +184,8 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
  * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
  * "batch_capture", "decide_spin", "blk_tile_cap", "blk_tile_order",
- * "trace_events"; "cu_count", "collect_rounds", "multi_passes" and
- * "blk_tiles" read-only).
+ * "plain_kernel", "trace_events"; "cu_count", "collect_rounds",
+ * "multi_passes", "blk_tiles" and "plain_blocked" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
  * 2^14 destinations with twice the mean in-edges) and dense_unreached
  * (default 0.5): an all-edge round whose frontier has at least
@@ -202,9 +202,14 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * square of the solve's round count); past it the call returns
  * SSSP_ERR_NOT_CONVERGED. "collect_rounds" (read-only) = rounds the last
  * collection ran.
- * plain_below_edges (default 2^25): a single-source heuristic or learned
+ * plain_below_edges (default 2^26): a single-source heuristic or learned
  * solve (or BFS) of a graph with fewer edges runs the plain pass loop, whose
- * passes then cost less than a round's fixed overhead.
+ * passes then cost less than the rounds with their fixed per-round cost.
+ * plain_kernel picks the plain pass: 0 (default) the destination-blocked pass
+ * (block minima in LDS, as the dense rounds) once the blocked list has 64
+ * tiles or more, else the XCD-partitioned edge-list pass; 1 always the
+ * edge-list pass, 2 always the blocked pass. "plain_blocked" (read-only) = 1
+ * when plain solves use the blocked pass.
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with
  * d >= (min distance over the frontier and pile) + delta and out-degree >=
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.

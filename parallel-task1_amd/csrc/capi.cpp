@@ -596,7 +596,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "batch_capture") {
             SSSP_REQUIRE(value == 0 || value == 1, "batch_capture must be 0 or 1");
             g->batch_capture = (int)value;
-
+        } else if (n == "plain_kernel") {
+            SSSP_REQUIRE(value == 0 || value == 1 || value == 2, "plain_kernel must be 0, 1 or 2");
+            g->plain_kernel = (int)value;
         } else if (n == "dense_unreached") {
             SSSP_REQUIRE(value >= 0 && value <= 2, "dense_unreached must be in [0, 2]");
             g->dense_unreached = value;
@@ -638,7 +640,8 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "blk_tile_cap") *value = (double)g->blk_tile_cap;
         else if (n == "blk_tile_order") *value = g->blk_tile_order;
         else if (n == "blk_tiles") *value = g->n_btile;
-
+        else if (n == "plain_kernel") *value = g->plain_kernel;
+        else if (n == "plain_blocked") *value = plain_uses_blocked(g) ? 1 : 0;
         else if (n == "multi_passes") *value = g->multi_passes;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;

@@ -1061,43 +1061,30 @@ __global__ __launch_bounds__(kBlock) void k_lowered_compact(const uint32_t *dist
     flush_acc(acc, slot);
 }
 
-// Dense all-edge round over the destination-blocked edge list. A workgroup
-// takes one tile: it loads its block's 2^kBlkBits distances into LDS, relaxes
-// the tile's edges into them (d[src] gathered from global memory, the minimum
-// taken with LDS atomics), then writes back the vertices it lowered and
-// appends them to the next frontier with the round's stamp. A block with one
-// tile belongs to one workgroup: plain stores, plain stamps, no global
-// atomics in the whole round. Tiles of a split block (a high in-degree
-// block) write back with a returning atomicMin and stamp with atomicExch, so
-// each lowered vertex is appended once. Fills the round's counters.
 constexpr int kBlkThreads = 1024;
-constexpr int kBlkStageCap = 128;  // per-wave LDS staging slots
+constexpr int kBlkPer = (1 << kBlkBits) / kBlkThreads;  // block vertices per thread
 
-template <class WT>
-__global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
-    const BlkTile *__restrict__ tiles, const int32_t *__restrict__ bsrc,
-    const uint16_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
-    int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, int32_t *qnext, RoundSlot *slot)
+// One tile of the destination-blocked edge list, shared by the dense round
+// and the plain blocked pass: loads the tile's block of 2^kBlkBits distances
+// into LDS (and into d0, the values as loaded, kept in registers for the
+// write-back: a second read of the block would cost 64 KB of HBM per tile),
+// then relaxes the tile's edges into it -- d[src] gathered from global
+// memory, the minimum taken with LDS atomics. Returns the number of relaxed
+// edges with a reached source (counted only when COUNT).
+template <class WT, bool COUNT>
+__device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t,
+                                                  const int32_t *__restrict__ bsrc,
+                                                  const uint16_t *__restrict__ bdst,
+                                                  const uint32_t *__restrict__ bw,
+                                                  const uint32_t *dist, int64_t base, int nb,
+                                                  uint32_t *s_d, uint32_t (&d0)[kBlkPer])
 {
     typedef int32_t v4i __attribute__((ext_vector_type(4)));
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
     typedef uint16_t v4h __attribute__((ext_vector_type(4)));
-    constexpr int NB = 1 << kBlkBits;
-    constexpr int PER = NB / kBlkThreads;  // block vertices per thread
     constexpr uint32_t INF = WT::INF;
-    __shared__ uint32_t s_d[NB];
-    __shared__ int32_t s_stage[kBlkThreads / 64][kBlkStageCap];
-    int32_t *wbuf = s_stage[threadIdx.x >> 6];
-    int wcnt = 0;
-    RoundAcc acc;
-    const BlkTile t = tiles[blockIdx.x];
-    const int64_t base = (int64_t)t.blk << kBlkBits;
-    const int nb = (int)min((int64_t)NB, V - base);
-    // the block's distances as loaded, kept in registers for the write-back
-    // (a second read of the block would cost 64 KB of HBM per tile)
-    uint32_t d0[PER];
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
+    for (int k = 0; k < kBlkPer; k++) {
         const int i = k * kBlkThreads + threadIdx.x;
         d0[k] = i < nb ? dist[base + i] : INF;
         s_d[i] = d0[k];
@@ -1106,11 +1093,12 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
     const v4i *s4 = reinterpret_cast<const v4i *>(bsrc);
     const v4h *t4 = reinterpret_cast<const v4h *>(bdst);
     const v4u *w4 = reinterpret_cast<const v4u *>(bw);
+    int64_t relaxed = 0;
     // branch-free: an unreached source gives INF, which lowers nothing; the
     // LDS minimum returns nothing, so no lane waits on it
     auto relax = [&](uint32_t du, uint32_t loc, uint32_t ww) {
         const bool fin = du != INF;
-        acc.relaxed += fin ? 1 : 0;
+        if constexpr (COUNT) relaxed += fin ? 1 : 0;
         atomicMin(&s_d[loc], fin ? WT::add(du, ww) : INF);
     };
     for (int64_t g = (t.e0 >> 2) + threadIdx.x; g < (t.e1 >> 2); g += kBlkThreads) {
@@ -1127,8 +1115,42 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
         relax(u3, tv.w, wv.w);
     }
     __syncthreads();
+    return relaxed;
+}
+
+// Dense all-edge round over the destination-blocked edge list. A workgroup
+// takes one tile (blk_relax_tile), then writes back the vertices it lowered
+// and appends them to the next frontier with the round's stamp. A block with
+// one tile belongs to one workgroup: plain stores, plain stamps, no global
+// atomics in the whole round. Tiles of a split block (a high in-degree
+// block) write back with a returning atomicMin and stamp with atomicExch, so
+// each lowered vertex is appended once. The workgroup reserves queue space
+// for all its appends with one atomicAdd: the append counts per (vertex row
+// k, wave) are scanned in LDS, and the vertices are written in slot order.
+// (Reserving per wave and flush, as the edge kernels do, kept every wave
+// waiting on a returning global atomic up to 16 times per tile: 0.3 ms of a
+// heavy round at R-MAT scale 25.) Fills the round's counters.
+template <class WT>
+__global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
+    const BlkTile *__restrict__ tiles, const int32_t *__restrict__ bsrc,
+    const uint16_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
+    int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, int32_t *qnext, RoundSlot *slot)
+{
+    constexpr int NW = kBlkThreads / 64;
+    static_assert(kBlkPer * NW == 256, "the append scan runs 4 counts per lane of one wave");
+    __shared__ uint32_t s_d[1 << kBlkBits];
+    __shared__ int32_t s_cnt[kBlkPer * NW];  // appends per (row k, wave), then their offsets
+    __shared__ int32_t s_qbase;
+    RoundAcc acc;
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    const BlkTile t = tiles[blockIdx.x];
+    const int64_t base = (int64_t)t.blk << kBlkBits;
+    const int nb = (int)min((int64_t)(1 << kBlkBits), V - base);
+    uint32_t d0[kBlkPer];
+    acc.relaxed = blk_relax_tile<WT, true>(t, bsrc, bdst, bw, dist, base, nb, s_d, d0);
+    uint32_t appm = 0;  // bit k: this lane appends its vertex of row k
 #pragma unroll
-    for (int k = 0; k < PER; k++) {  // block-uniform: every lane reaches stage_push
+    for (int k = 0; k < kBlkPer; k++) {  // block-uniform: every lane reaches the ballot
         const int i = k * kBlkThreads + threadIdx.x;
         const int32_t v = (int32_t)(base + i);
         bool app = false;
@@ -1157,11 +1179,87 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
                 }
             }
         }
-        stage_push(app, v, wbuf, wcnt);
-        if (wcnt > kBlkStageCap - 64) stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+        appm |= (uint32_t)app << k;
+        const uint64_t m = __ballot(app);
+        if (lane == 0) s_cnt[k * NW + w] = __popcll(m);
     }
-    stage_flush(wbuf, wcnt, qnext, &slot->qlen);
-    flush_acc_n<kBlkThreads / 64>(acc, slot);
+    __syncthreads();
+    if (w == 0) {
+        // exclusive scan of the 256 counts in (k, wave) order: 4 per lane
+        const int c0 = s_cnt[4 * lane], c1 = s_cnt[4 * lane + 1], c2 = s_cnt[4 * lane + 2],
+                  c3 = s_cnt[4 * lane + 3];
+        const int sum = c0 + c1 + c2 + c3;
+        int incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const int ex = incl - sum;
+        s_cnt[4 * lane] = ex;
+        s_cnt[4 * lane + 1] = ex + c0;
+        s_cnt[4 * lane + 2] = ex + c0 + c1;
+        s_cnt[4 * lane + 3] = ex + c0 + c1 + c2;
+        if (lane == 63) s_qbase = incl > 0 ? atomicAdd(&slot->qlen, incl) : 0;
+    }
+    __syncthreads();
+    if (appm) {
+        const int32_t qb = s_qbase;
+#pragma unroll
+        for (int k = 0; k < kBlkPer; k++) {
+            const bool app = (appm >> k) & 1u;
+            const uint64_t m = __ballot(app);
+            if (app)
+                qnext[qb + s_cnt[k * NW + w] + __popcll(m & lanemask_lt())] =
+                    (int32_t)(base + k * kBlkThreads + threadIdx.x);
+        }
+    }
+    flush_acc_n<NW>(acc, slot);
+}
+
+// Plain Bellman-Ford pass (bellman_ford.cu:62-99) over the destination-blocked
+// edge list: the tile relaxation of the dense round without its frontier
+// (no stamps, no appends, no counters), and the change flag protocol of
+// k_el_pass -- pass p zeroes the flag of pass p+1, returns at once when pass
+// p-1 changed nothing, and sets its own flag when it lowers a distance -- so
+// passes are queued without a host round trip. A tile that finds a vertex
+// below the value it loaded in this pass has seen a distance fall in this
+// pass (whoever wrote it last), so the flag never reports a change that did
+// not happen, and a pass that leaves it clear relaxed every edge against the
+// final distances: the fixpoint.
+template <class WT>
+__global__ __launch_bounds__(kBlkThreads) void k_blk_plain(
+    const BlkTile *__restrict__ tiles, const int32_t *__restrict__ bsrc,
+    const uint16_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
+    int32_t *changed, int32_t pass)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) changed[(pass + 1) & 63] = 0;
+    if (pass > 0 && __hip_atomic_load(&changed[(pass - 1) & 63], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == 0)
+        return;
+    __shared__ uint32_t s_d[1 << kBlkBits];
+    const BlkTile t = tiles[blockIdx.x];
+    const int64_t base = (int64_t)t.blk << kBlkBits;
+    const int nb = (int)min((int64_t)(1 << kBlkBits), V - base);
+    uint32_t d0[kBlkPer];
+    blk_relax_tile<WT, false>(t, bsrc, bdst, bw, dist, base, nb, s_d, d0);
+    bool low = false;
+#pragma unroll
+    for (int k = 0; k < kBlkPer; k++) {
+        const int i = k * kBlkThreads + threadIdx.x;
+        if (i < nb) {
+            const uint32_t nv = s_d[i];
+            if (nv < d0[k]) {
+                low = true;
+                if (t.split)  // the result of the min is not needed
+                    __hip_atomic_fetch_min(&dist[base + i], nv, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    dist[base + i] = nv;  // only this workgroup writes the block
+            }
+        }
+    }
+    if (__ballot(low) != 0 && __lane_id() == 0) changed[pass & 63] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1506,6 +1604,7 @@ void solve_plain(sssp_graph *g, sssp_stats *st)
     int64_t nblocks = (int64_t)g->cu_count * 8;
     if (nblocks < 8) nblocks = 8;
     const int64_t per_block = (ng + nblocks - 1) / nblocks;
+    const bool blocked = plain_uses_blocked(g);
     int32_t pass = 0;
     // with trace_events every pass is bracketed by events (one pass per batch)
     const int batch = g->trace_events ? 1 : 4;
@@ -1513,10 +1612,17 @@ void solve_plain(sssp_graph *g, sssp_stats *st)
     while (done_at < 0) {
         for (int i = 0; i < batch; i++) {
             if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
-            k_el_pass<WT, uint32_t, 4, true><<<(unsigned)nblocks, kBlock, 0, s>>>(
-                g->x_src, g->x_dst, g->x_w, E, g->dist, g->ctrl->changed, pass + i, per_block,
-                g->xgoff);
-            SSSP_LAUNCH_CHECK("k_el_pass");
+            if (blocked) {
+                k_blk_plain<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, s>>>(
+                    g->b_tile, g->b_src, g->b_dst, g->b_w, g->dist, g->V, g->ctrl->changed,
+                    pass + i);
+                SSSP_LAUNCH_CHECK("k_blk_plain");
+            } else {
+                k_el_pass<WT, uint32_t, 4, true><<<(unsigned)nblocks, kBlock, 0, s>>>(
+                    g->x_src, g->x_dst, g->x_w, E, g->dist, g->ctrl->changed, pass + i,
+                    per_block, g->xgoff);
+                SSSP_LAUNCH_CHECK("k_el_pass");
+            }
             if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r1, s));
         }
         SSSP_HIP(hipMemcpyAsync(g->ctrl_host->changed, g->ctrl->changed, sizeof(int32_t) * 64,
@@ -1718,9 +1824,12 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
         SSSP_LAUNCH_CHECK("k_init_sources");
     }
     // Below plain_below_edges the rule and the network hand the whole solve
-    // to the plain pass loop: a pass then costs less than a round's fixed
-    // host read-back and small-kernel launches (measured crossover between
-    // R-MAT scale 20 and 22 at edge factor 16).
+    // to the plain pass loop: its passes (the blocked pass from 64 tiles)
+    // then cost less than the rounds with their fixed per-round cost.
+    // Measured (learned, ms): R-MAT 21 x 16 (2^25 edges) 2.98 plain against
+    // 3.66 in rounds, BFS 0.79 against 0.95; at 2^26 edges about even (R-MAT
+    // 22 x 16 5.12 / 5.00, R-MAT 21 x 32 4.46 / 4.71); R-MAT 22 x 32 (2^27)
+    // 7.1 against 5.6 in rounds.
     const bool small = (sched == SSSP_SCHED_HEURISTIC || sched == SSSP_SCHED_LEARNED) &&
                        n_src == 1 && g->E < g->plain_below_edges;
     if (sched == SSSP_SCHED_PLAIN || small) {

@@ -221,11 +221,12 @@ struct sssp_graph {
     double pull_frac = 0.5;        // heuristic: pull when unsettled in-edges < pull_frac * frontier edges
     double delta = 0.0;            // push rounds defer vertices with d >= min + delta (0 = off)
     int hub_degree = 0;            // ... unless their out-degree is below this
-    int64_t plain_below_edges = (int64_t)1 << 25;  // heuristic/learned: plain passes below this E
+    int64_t plain_below_edges = (int64_t)1 << 26;  // heuristic/learned: plain passes below this E
     double dense_all_frac = 0.15;  // all-edge rounds with frontier edges >= this * E run dense
                                    // (0.4 when the layout build finds no hub blocks),
     double dense_unreached = 0.5;  // ... and so do those with this sampled unreached fraction
     int dense_kernel = 0;  // dense rounds: 0 = destination-blocked pass, 1 = pass + compaction
+    int plain_kernel = 0;  // plain passes: 0 = by tile count, 1 = edge-list pass, 2 = blocked pass
     bool last_round_dense = false;  // the last round ran dense (traces)
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds
@@ -266,6 +267,24 @@ struct sssp_graph {
     int32_t *bround = nullptr;    // per-workgroup round counters
     int32_t bround_cap = 0;
 };
+
+namespace sssp {
+// The plain pass kernel a graph uses: the destination-blocked pass once the
+// blocked list has at least kPlainBlkMinTiles tiles (one 1024-thread
+// workgroup each), else the XCD-partitioned edge-list pass, whose grid fills
+// the chip whatever the graph's block structure. Measured crossover (plain
+// solve, R-MAT edge factor 16): 44 tiles (scale 18) 1.08 against 0.96 ms for
+// the edge-list pass, 83 tiles (scale 19) 1.27 against 1.57 ms; R-MAT
+// 22 x 32 (1196 tiles) 7.1 against 15.6 ms.
+constexpr int64_t kPlainBlkMinTiles = 64;
+
+inline bool plain_uses_blocked(const sssp_graph *g)
+{
+    if (g->plain_kernel == 1 || g->n_btile == 0) return false;
+    if (g->plain_kernel == 2) return true;
+    return g->n_btile >= kPlainBlkMinTiles;
+}
+}  // namespace sssp
 
 namespace sssp {
 

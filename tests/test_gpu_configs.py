@@ -84,11 +84,15 @@ def test_c2_rmat22_ef32_sixteen_sources(gpu, oracle):
         assert E == 134217728
         m = _with_network(g)
         sources = S.pick_sources(src, V, 2, 16)
-        t = {"learned": [], "heuristic": [], "plain": []}
+        # plain = the default plain pass kernel (the destination-blocked pass
+        # here), plain_edge_list = the same passes over the edge list
+        assert g.get_param("plain_blocked") == 1
+        t = {"learned": [], "heuristic": [], "plain": [], "plain_edge_list": []}
         for s in sources:
             ref = None
             for sched in t:
-                d, st = g.solve(int(s), sched)
+                g.set_param("plain_kernel", 1 if sched == "plain_edge_list" else 0)
+                d, st = g.solve(int(s), "plain" if sched == "plain_edge_list" else sched)
                 t[sched].append(st["t_solve_ms"])
                 if ref is None:
                     ref = d

@@ -271,16 +271,19 @@ def test_batch_captured_launch_replays(gpu, oracle):
 
 
 @pytest.mark.gpu
-def test_plain_solve_queued_and_traced(gpu, oracle):
+@pytest.mark.parametrize("kernel", [1, 2], ids=["edge_list", "blocked"])
+def test_plain_solve_queued_and_traced(gpu, oracle, kernel):
     """The plain schedule queues its passes four at a time (change flags tested
     on the device); with trace_events each pass is launched alone between
     events. Same distances; the traced solve records one row per pass, the
     last one a pass that changed nothing; a pass cap below the fixpoint gives
-    up and leaves the graph usable."""
+    up and leaves the graph usable. Both plain pass kernels."""
     src, dst, w = S.rmat(15, 16, seed=42, threads=16)
     V = 1 << 15
     ref = oracle.dijkstra(src, dst, w, V, 7)
     with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_kernel", kernel)
+        assert g.get_param("plain_blocked") == (kernel == 2)
         d, st = g.solve(7, "plain")
         assert np.array_equal(bits(d), bits(ref))
         assert 2 <= st["iterations"] <= V + 1 and st["verify_passes"] == 1
@@ -298,6 +301,70 @@ def test_plain_solve_queued_and_traced(gpu, oracle):
         g.set_param("max_rounds", 100000)
         d3, _ = g.solve(7, "plain")
         assert np.array_equal(bits(d3), bits(ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", [1, 2], ids=["edge_list", "blocked"])
+def test_plain_kernels_bit_exact(gpu, oracle, kernel):
+    """The plain schedule on either pass kernel -- the XCD-partitioned edge
+    list, or the destination-blocked list (block minima in LDS, atomicMin
+    write-backs for blocks split over several tiles) -- ends on the oracle's
+    fixpoint: f32 with vertex ids and with degree-ordered slots, u32, BFS
+    levels, and a graph whose hub block is split, with tiles down to 256
+    edges."""
+    src, dst, w = S.rmat(14, 16, seed=9, threads=64)
+    V = 1 << 14
+    sources = [0] + list(S.pick_sources(src, V, 109, 3))
+    for relabel in (None, True):
+        with S.DeviceGraph(src, dst, w, V, relabel=relabel) as g:
+            g.set_param("plain_kernel", kernel)
+            for s in sources:
+                print(f"plain kernel {kernel}: rmat14 relabel {relabel} source {s}", flush=True)
+                ref, _ = oracle.bf(src, dst, w, V, s)
+                d, st = g.solve(s, "plain")
+                assert np.array_equal(bits(d), bits(ref)), (s, st)
+                assert st["verify_passes"] == 1
+                ok, counts, _ = g.certify(s, parents=False)
+                assert ok, counts
+    V = 1 << 15
+    src, dst, wu = S.er(V, 8, seed=3, weight_mode=1)
+    with S.DeviceGraph(src, dst, wu, V) as g:
+        g.set_param("plain_kernel", kernel)
+        print(f"plain kernel {kernel}: er15 u32", flush=True)
+        d, _ = g.solve(0, "plain")
+        assert np.array_equal(d, oracle.dijkstra(src, dst, wu, V, 0))
+    scale = 17
+    src, dst, w = S.rmat(scale, 16, seed=44, threads=16)
+    V = 1 << scale
+    assert np.bincount(dst >> 14).max() > (1 << 19)  # the hub block is split
+    with S.DeviceGraph(src, dst, w, V, relabel=True) as g:
+        g.set_param("plain_kernel", kernel)
+        for cap in (256, 1 << 17):
+            g.set_param("blk_tile_cap", cap)
+            for s in (0, int(src[999])):
+                print(f"plain kernel {kernel}: rmat17 split blocks, cap {cap}, source {s}", flush=True)
+                d, _ = g.solve(s, "plain")
+                assert np.array_equal(bits(d), bits(oracle.dijkstra(src, dst, w, V, s))), (cap, s)
+                _, lev, _ = g.bfs(s, "plain")
+                assert np.array_equal(lev, oracle.bfs(src, dst, V, s)[1]), (cap, s)
+
+
+@pytest.mark.gpu
+def test_plain_kernel_choice(gpu):
+    """plain_kernel 0 picks the blocked pass once the blocked list has 64
+    tiles or more (here forced by a small tile cap), else the edge-list
+    pass; 1 and 2 force either; other values are rejected."""
+    src, dst, w = S.rmat(14, 16, seed=9, threads=64)
+    with S.DeviceGraph(src, dst, w, 1 << 14) as g:
+        assert g.get_param("plain_kernel") == 0
+        assert g.get_param("blk_tiles") < 64 and g.get_param("plain_blocked") == 0
+        g.set_param("blk_tile_cap", 2048)
+        assert g.get_param("blk_tiles") >= 64 and g.get_param("plain_blocked") == 1
+        g.set_param("plain_kernel", 1)
+        assert g.get_param("plain_blocked") == 0
+        for bad in (-1, 3):
+            with pytest.raises(S.SSSPError):
+                g.set_param("plain_kernel", bad)
 
 
 @pytest.mark.gpu
@@ -555,7 +622,7 @@ def test_small_graphs_hand_the_rule_and_network_to_plain_passes(gpu, oracle):
     V = 1 << 13
     ref = oracle.dijkstra(src, dst, w, V, 0)
     with S.DeviceGraph(src, dst, w, V) as g:
-        assert g.get_param("plain_below_edges") == 2 ** 25
+        assert g.get_param("plain_below_edges") == 2 ** 26
         for sched in ("heuristic", "learned"):
             d, st = g.solve(0, sched)
             assert np.array_equal(bits(d), bits(ref)), sched
