@@ -45,7 +45,7 @@ void graph_free(sssp_graph *g)
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
                     g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap,
-                    g->b_src, g->b_dst, g->b_w, g->b_tile, g->m_rec, g->m_tile, g->mdist,
+                    g->b_pk, g->b_cbase, g->b_wide, g->b_w, g->b_tile, g->m_rec, g->m_tile, g->mdist,
                     g->vperm, g->vinv, g->dout != g->dist ? g->dout : nullptr, g->msrc, g->mflags};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (auto &pair : g->mexec)
@@ -215,7 +215,8 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     // queues, load-balancing entries, static entry lists)
     const int64_t Ei = g->E, Vi = g->V;
     const int64_t xe = g->xgoff_host.empty() ? 0 : g->xgoff_host.back();
-    g->device_bytes = Ei * 12 + xe * 12 + g->b_edges * 10 + (int64_t)g->n_btile * 24 +
+    g->device_bytes = Ei * 12 + xe * 12 + g->b_edges * 8 + g->b_edges / kBlkChunk * 4 +
+                      g->b_nwide * kBlkChunk * 4 + (int64_t)g->n_btile * 24 +
                       (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) + Vi * 4 * 7 +
                       (Vi + 1) * 4 * 3 + (Vi + 1) * 8 * 2 + Vi * 24 + (g->relabel ? Vi * 12 : 0);
 }
@@ -583,8 +584,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
             SSSP_REQUIRE(value == 0 || value == 1, "dense_kernel must be 0 or 1");
             g->dense_kernel = (int)value;
         } else if (n == "blk_tile_cap") {
-            SSSP_REQUIRE(value >= 256 && value <= 2147483648.0 && std::fmod(value, 4.0) == 0,
-                         "blk_tile_cap must be a multiple of 4 in [256, 2^31]");
+            SSSP_REQUIRE(value >= (double)kBlkChunk && value <= 2147483648.0 &&
+                             std::fmod(value, (double)kBlkChunk) == 0,
+                         "blk_tile_cap must be a multiple of 256 in [256, 2^31]");
             retile_blocked(g, (int64_t)value, g->blk_tile_order);
         } else if (n == "blk_tile_order") {
             SSSP_REQUIRE(value == 0 || value == 1 || value == 2 || value == 3,
@@ -640,6 +642,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "blk_tile_cap") *value = (double)g->blk_tile_cap;
         else if (n == "blk_tile_order") *value = g->blk_tile_order;
         else if (n == "blk_tiles") *value = g->n_btile;
+        else if (n == "blk_wide_chunks") *value = (double)g->b_nwide;
         else if (n == "plain_kernel") *value = g->plain_kernel;
         else if (n == "plain_blocked") *value = plain_uses_blocked(g) ? 1 : 0;
         else if (n == "multi_passes") *value = g->multi_passes;

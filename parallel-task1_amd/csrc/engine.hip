@@ -1066,33 +1066,47 @@ constexpr int kBlkPer = (1 << kBlkBits) / kBlkThreads;  // block vertices per th
 
 // One tile of the destination-blocked edge list, shared by the dense round
 // and the plain blocked pass: loads the tile's block of 2^kBlkBits distances
-// into LDS (and into d0, the values as loaded, kept in registers for the
-// write-back: a second read of the block would cost 64 KB of HBM per tile),
-// then relaxes the tile's edges into it -- d[src] gathered from global
-// memory, the minimum taken with LDS atomics. Returns the number of relaxed
-// edges with a reached source (counted only when COUNT).
+// into LDS, then relaxes the tile's edges into it -- d[src] gathered from
+// global memory, the minimum taken with LDS atomics. The callers compare the
+// LDS minima with the block read again from global memory (64 KB per tile,
+// 2 % of a round's bytes at R-MAT scale 25): keeping the loaded values in
+// registers instead took 16 VGPRs per lane, which left room for one
+// 1024-thread workgroup per CU; with two per CU (8 waves per SIMD, 64 VGPRs
+// at most) a dense round at R-MAT scale 25 takes 1.04-1.24 ms instead of
+// 1.29-1.35 and a plain blocked pass 0.98-1.09 instead of 1.10-1.19. A wave takes one chunk of 256
+// edge slots per step, 4 per lane: the sources are the chunk's first source
+// plus a wave prefix sum of the packed deltas (or, for a chunk with a large
+// gap, read whole). Returns the number of relaxed edges with a reached source
+// (counted only when COUNT).
+struct BlkList {
+    const uint32_t *pk;     // packed edges: source delta << kBlkBits | destination offset
+    const int32_t *cbase;   // per chunk: first source, or -(1 + index) of whole sources
+    const int32_t *wide;    // whole sources of the chunks that need them
+    const uint32_t *w;      // weights
+};
+
+BlkList blk_list(const sssp_graph *g) { return BlkList{g->b_pk, g->b_cbase, g->b_wide, g->b_w}; }
+
 template <class WT, bool COUNT>
-__device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t,
-                                                  const int32_t *__restrict__ bsrc,
-                                                  const uint16_t *__restrict__ bdst,
-                                                  const uint32_t *__restrict__ bw,
+__device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t, const BlkList L,
                                                   const uint32_t *dist, int64_t base, int nb,
-                                                  uint32_t *s_d, uint32_t (&d0)[kBlkPer])
+                                                  uint32_t *s_d)
 {
     typedef int32_t v4i __attribute__((ext_vector_type(4)));
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-    typedef uint16_t v4h __attribute__((ext_vector_type(4)));
     constexpr uint32_t INF = WT::INF;
+    constexpr uint32_t kOffMask = (1u << kBlkBits) - 1;
+    static_assert(kBlkChunk == 4 * 64, "a chunk is 4 slots per lane of one wave");
 #pragma unroll
     for (int k = 0; k < kBlkPer; k++) {
         const int i = k * kBlkThreads + threadIdx.x;
-        d0[k] = i < nb ? dist[base + i] : INF;
-        s_d[i] = d0[k];
+        s_d[i] = i < nb ? dist[base + i] : INF;
     }
     __syncthreads();
-    const v4i *s4 = reinterpret_cast<const v4i *>(bsrc);
-    const v4h *t4 = reinterpret_cast<const v4h *>(bdst);
-    const v4u *w4 = reinterpret_cast<const v4u *>(bw);
+    const v4u *p4 = reinterpret_cast<const v4u *>(L.pk);
+    const v4u *w4 = reinterpret_cast<const v4u *>(L.w);
+    const v4i *x4 = reinterpret_cast<const v4i *>(L.wide);
+    const int lane = __lane_id();
     int64_t relaxed = 0;
     // branch-free: an unreached source gives INF, which lowers nothing; the
     // LDS minimum returns nothing, so no lane waits on it
@@ -1101,18 +1115,44 @@ __device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t,
         if constexpr (COUNT) relaxed += fin ? 1 : 0;
         atomicMin(&s_d[loc], fin ? WT::add(du, ww) : INF);
     };
+    // tiles hold whole chunks and the wave's 64 lanes cover one chunk per
+    // step, so the loop bound and the chunk index are wave-uniform
     for (int64_t g = (t.e0 >> 2) + threadIdx.x; g < (t.e1 >> 2); g += kBlkThreads) {
-        const v4i sv = __builtin_nontemporal_load(&s4[g]);
-        const v4h tv = __builtin_nontemporal_load(&t4[g]);
+        const int32_t cb = L.cbase[g >> 6];
+        const v4u pv = __builtin_nontemporal_load(&p4[g]);
         const v4u wv = __builtin_nontemporal_load(&w4[g]);
+        int32_t s0, s1, s2, s3;
+        if (cb >= 0) {
+            const int32_t a0 = (int32_t)(pv.x >> kBlkBits);
+            const int32_t a1 = a0 + (int32_t)(pv.y >> kBlkBits);
+            const int32_t a2 = a1 + (int32_t)(pv.z >> kBlkBits);
+            const int32_t a3 = a2 + (int32_t)(pv.w >> kBlkBits);
+            int32_t incl = a3;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            const int32_t b0 = cb + (incl - a3);
+            s0 = b0 + a0;
+            s1 = b0 + a1;
+            s2 = b0 + a2;
+            s3 = b0 + a3;
+        } else {
+            const v4i sv = x4[(int64_t)(-1 - cb) * (kBlkChunk / 4) + lane];
+            s0 = sv.x;
+            s1 = sv.y;
+            s2 = sv.z;
+            s3 = sv.w;
+        }
         // the four gathers are issued together (equal sources, frequent in
         // the src-sorted runs, hit the same line); a reuse test between them
         // would serialise them behind one another's latency
-        const uint32_t u0 = dist[sv.x], u1 = dist[sv.y], u2 = dist[sv.z], u3 = dist[sv.w];
-        relax(u0, tv.x, wv.x);
-        relax(u1, tv.y, wv.y);
-        relax(u2, tv.z, wv.z);
-        relax(u3, tv.w, wv.w);
+        const uint32_t u0 = dist[s0], u1 = dist[s1], u2 = dist[s2], u3 = dist[s3];
+        relax(u0, pv.x & kOffMask, wv.x);
+        relax(u1, pv.y & kOffMask, wv.y);
+        relax(u2, pv.z & kOffMask, wv.z);
+        relax(u3, pv.w & kOffMask, wv.w);
     }
     __syncthreads();
     return relaxed;
@@ -1131,9 +1171,8 @@ __device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t,
 // waiting on a returning global atomic up to 16 times per tile: 0.3 ms of a
 // heavy round at R-MAT scale 25.) Fills the round's counters.
 template <class WT>
-__global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
-    const BlkTile *__restrict__ tiles, const int32_t *__restrict__ bsrc,
-    const uint16_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
+__global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_blk_pass(
+    const BlkTile *__restrict__ tiles, const BlkList L, uint32_t *dist, int64_t V,
     int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, int32_t *qnext, RoundSlot *slot)
 {
     constexpr int NW = kBlkThreads / 64;
@@ -1146,8 +1185,7 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
     const BlkTile t = tiles[blockIdx.x];
     const int64_t base = (int64_t)t.blk << kBlkBits;
     const int nb = (int)min((int64_t)(1 << kBlkBits), V - base);
-    uint32_t d0[kBlkPer];
-    acc.relaxed = blk_relax_tile<WT, true>(t, bsrc, bdst, bw, dist, base, nb, s_d, d0);
+    acc.relaxed = blk_relax_tile<WT, true>(t, L, dist, base, nb, s_d);
     uint32_t appm = 0;  // bit k: this lane appends its vertex of row k
 #pragma unroll
     for (int k = 0; k < kBlkPer; k++) {  // block-uniform: every lane reaches the ballot
@@ -1155,11 +1193,12 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
         const int32_t v = (int32_t)(base + i);
         bool app = false;
         if (i < nb) {
-            // below the loaded value: this tile lowered v. For a split block
-            // another tile may have lowered it further since; the returning
-            // atomicMin then reports no change.
+            // below the value in global memory: this tile lowers v (for a
+            // block of one tile that value is the one loaded; for a split
+            // block another tile may lower it further before the returning
+            // atomicMin, which then reports no change)
             const uint32_t nv = s_d[i];
-            if (nv < d0[k]) {
+            if (nv < dist[v]) {
                 bool low = true;
                 if (t.split) low = atomicMin(&dist[v], nv) > nv;
                 else dist[v] = nv;  // only this workgroup writes the block
@@ -1222,15 +1261,14 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_pass(
 // (no stamps, no appends, no counters), and the change flag protocol of
 // k_el_pass -- pass p zeroes the flag of pass p+1, returns at once when pass
 // p-1 changed nothing, and sets its own flag when it lowers a distance -- so
-// passes are queued without a host round trip. A tile that finds a vertex
-// below the value it loaded in this pass has seen a distance fall in this
-// pass (whoever wrote it last), so the flag never reports a change that did
-// not happen, and a pass that leaves it clear relaxed every edge against the
-// final distances: the fixpoint.
+// passes are queued without a host round trip. A tile sets the flag only
+// when its minimum for a vertex is below that vertex's distance in global
+// memory, which its write then lowers: the flag never reports a change that
+// did not happen, and a pass that leaves it clear relaxed every edge against
+// the final distances: the fixpoint.
 template <class WT>
-__global__ __launch_bounds__(kBlkThreads) void k_blk_plain(
-    const BlkTile *__restrict__ tiles, const int32_t *__restrict__ bsrc,
-    const uint16_t *__restrict__ bdst, const uint32_t *__restrict__ bw, uint32_t *dist, int64_t V,
+__global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_blk_plain(
+    const BlkTile *__restrict__ tiles, const BlkList L, uint32_t *dist, int64_t V,
     int32_t *changed, int32_t pass)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) changed[(pass + 1) & 63] = 0;
@@ -1241,15 +1279,14 @@ __global__ __launch_bounds__(kBlkThreads) void k_blk_plain(
     const BlkTile t = tiles[blockIdx.x];
     const int64_t base = (int64_t)t.blk << kBlkBits;
     const int nb = (int)min((int64_t)(1 << kBlkBits), V - base);
-    uint32_t d0[kBlkPer];
-    blk_relax_tile<WT, false>(t, bsrc, bdst, bw, dist, base, nb, s_d, d0);
+    blk_relax_tile<WT, false>(t, L, dist, base, nb, s_d);
     bool low = false;
 #pragma unroll
     for (int k = 0; k < kBlkPer; k++) {
         const int i = k * kBlkThreads + threadIdx.x;
         if (i < nb) {
             const uint32_t nv = s_d[i];
-            if (nv < d0[k]) {
+            if (nv < dist[base + i]) {
                 low = true;
                 if (t.split)  // the result of the min is not needed
                     __hip_atomic_fetch_min(&dist[base + i], nv, __ATOMIC_RELAXED,
@@ -1495,7 +1532,7 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
             g->last_round_dense = true;
             if (g->dense_kernel == 0 && g->n_btile > 0) {
                 k_blk_pass<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, s>>>(
-                    g->b_tile, g->b_src, g->b_dst, g->b_w, g->dist, g->V, g->stamp, a.tick_cur,
+                    g->b_tile, blk_list(g), g->dist, g->V, g->stamp, a.tick_cur,
                     g->row_ptr, a.qnext, a.slot);
                 SSSP_LAUNCH_CHECK("k_blk_pass");
                 return false;
@@ -1614,7 +1651,7 @@ void solve_plain(sssp_graph *g, sssp_stats *st)
             if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
             if (blocked) {
                 k_blk_plain<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, s>>>(
-                    g->b_tile, g->b_src, g->b_dst, g->b_w, g->dist, g->V, g->ctrl->changed,
+                    g->b_tile, blk_list(g), g->dist, g->V, g->ctrl->changed,
                     pass + i);
                 SSSP_LAUNCH_CHECK("k_blk_plain");
             } else {

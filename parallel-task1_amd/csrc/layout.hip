@@ -347,21 +347,78 @@ __global__ __launch_bounds__(kBlock) void k_blk_gather(const uint32_t *skeys, co
     }
 }
 
-// Up to 3 padding slots per block: weight-0 self loops on the block's first
-// vertex (a candidate equal to the distance it would lower: no change).
+// The padding slots of each block (up to kBlkChunk - 1) copy the block's last
+// edge: relaxing it again changes nothing, and its source delta is 0.
 __global__ __launch_bounds__(kBlock) void k_blk_pad(const int64_t *bptr, const int64_t *boff,
                                                     int32_t nblk, int32_t *bs, uint16_t *bd,
                                                     uint32_t *bw)
 {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= (int64_t)nblk * 4) return;
-    const int32_t b = (int32_t)(i >> 2);
-    const int64_t o = boff[b] + (bptr[b + 1] - bptr[b]) + (i & 3);
-    if (o < boff[b + 1]) {
-        bs[o] = b << kBlkBits;
-        bd[o] = 0;
-        bw[o] = 0u;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < (int64_t)nblk * kBlkChunk;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int32_t b = (int32_t)(i / kBlkChunk);
+        const int64_t last = boff[b] + (bptr[b + 1] - bptr[b]) - 1;
+        const int64_t o = last + 1 + i % kBlkChunk;
+        if (o < boff[b + 1]) {  // only blocks with edges have slots
+            bs[o] = bs[last];
+            bd[o] = bd[last];
+            bw[o] = bw[last];
+        }
     }
+}
+
+// Source deltas of one chunk (one wave, 4 slots per lane, in slot order):
+// d[k] = source of slot k - source of the slot before it, 0 for the first.
+__device__ __forceinline__ void chunk_deltas(const int32_t *bs, int64_t c, int32_t (&s)[4],
+                                             uint32_t (&d)[4])
+{
+    typedef int32_t v4i __attribute__((ext_vector_type(4)));
+    const v4i v = reinterpret_cast<const v4i *>(bs)[c * (kBlkChunk / 4) + __lane_id()];
+    s[0] = v.x;
+    s[1] = v.y;
+    s[2] = v.z;
+    s[3] = v.w;
+    const int32_t prev = __shfl_up(s[3], 1);
+    d[0] = __lane_id() == 0 ? 0u : (uint32_t)(s[0] - prev);
+    d[1] = (uint32_t)(s[1] - s[0]);
+    d[2] = (uint32_t)(s[2] - s[1]);
+    d[3] = (uint32_t)(s[3] - s[2]);
+}
+
+// wide[c] = 1 when chunk c has a source gap that does not fit the packed
+// delta (sources ascend within a block, so every delta is >= 0)
+__global__ __launch_bounds__(kBlock) void k_blk_chunk_wide(const int32_t *bs, int64_t nch,
+                                                           uint8_t *wide)
+{
+    const int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (c >= nch) return;  // wave-uniform
+    int32_t s[4];
+    uint32_t d[4];
+    chunk_deltas(bs, c, s, d);
+    bool big = false;
+    for (int k = 0; k < 4; k++) big |= d[k] >= (1u << kBlkDeltaBits);
+    const uint64_t m = __ballot(big);
+    if (__lane_id() == 0) wide[c] = m != 0;
+}
+
+// Packs chunk c: widx[c] < 0 -> deltas in the words and the first source in
+// cbase; else the destinations alone in the words, the sources whole in
+// wide[widx[c] * kBlkChunk ...] and cbase = -(1 + widx[c]).
+__global__ __launch_bounds__(kBlock) void k_blk_pack(const int32_t *bs, const uint16_t *bd,
+                                                     const int32_t *widx, int64_t nch,
+                                                     uint32_t *pk, int32_t *cbase, int32_t *wide)
+{
+    const int64_t c = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (c >= nch) return;  // wave-uniform
+    int32_t s[4];
+    uint32_t d[4];
+    chunk_deltas(bs, c, s, d);
+    const int32_t wi = widx[c];
+    const int64_t o = c * kBlkChunk + 4 * __lane_id();
+    for (int k = 0; k < 4; k++)
+        pk[o + k] = (wi < 0 ? d[k] << kBlkBits : 0u) | (uint32_t)bd[o + k];
+    if (wi >= 0)
+        for (int k = 0; k < 4; k++) wide[(int64_t)wi * kBlkChunk + 4 * __lane_id() + k] = s[k];
+    if (__lane_id() == 0) cbase[c] = wi < 0 ? s[0] : -(1 + wi);
 }
 
 __global__ __launch_bounds__(kBlock) void k_m_keys(const int32_t *dst, int64_t E, uint32_t *keys)
@@ -679,8 +736,8 @@ void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int w
     if (hb[1]) throw Error{SSSP_ERR_ARG, std::to_string(hb[1]) + " edges have a negative or NaN weight"};
 }
 
-// The tiles of the destination-blocked list: runs of at most `cap` edges
-// (a multiple of 4) of one block, `split` when a block has several; order 0 =
+// The tiles of the destination-blocked list: runs of at most `cap` edge slots
+// (a multiple of kBlkChunk) of one block, `split` when a block has several; order 0 =
 // smallest tiles first (stable by block), 1 = by block, 2 = largest first,
 // 3 = by descending block. bo: padded block offsets, nblk + 1 values.
 std::vector<BlkTile> make_blk_tiles(const std::vector<int64_t> &bo, int64_t cap, int order)
@@ -940,14 +997,15 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
             max_blk = std::max(max_blk, bp[(size_t)b + 1] - bp[(size_t)b]);
         if ((double)max_blk * nblk < 2.0 * (double)E) g->dense_all_frac = 0.4;
         for (int32_t b = 0; b < nblk; b++)
-            bo[(size_t)b + 1] = bo[(size_t)b] + (bp[(size_t)b + 1] - bp[(size_t)b] + 3) / 4 * 4;
+            bo[(size_t)b + 1] = bo[(size_t)b] +
+                                (bp[(size_t)b + 1] - bp[(size_t)b] + kBlkChunk - 1) / kBlkChunk * kBlkChunk;
         // Tile cap: 2^17 edges below 2^29 edges, doubled while kBlkTileCount
         // tiles of the cap would not exceed the graph -- the 2.08 G-edge R-MAT
         // graph ran 50.9 GTEPS with 2^19-edge tiles, 46.7 with 2^17; with
         // degree-ordered slots R-MAT 24 x 16 ran 11.3 / 11.4 ms (learned) at
         // 2^17 / 2^18 and R-MAT 25 x 16 (2^29 edges) 19.6 / 19.2 ms, its
-        // dense round 1.58 / 1.43 ms. A power of two, so every tile starts on
-        // a 4-edge group.
+        // dense round 1.58 / 1.43 ms. A power of two, so every tile holds
+        // whole chunks.
         int64_t tcap = kBlkTileEdges;
         while (tcap * kBlkTileCount <= E) tcap <<= 1;
         g->b_off_host = bo;
@@ -955,9 +1013,20 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         g->blk_tile_cap = tcap;
         g->blk_tile_order = 0;
         const int64_t Ep = bo[(size_t)nblk];
+        const int64_t nch = Ep / kBlkChunk;
         SSSP_HIP(hipMemcpyAsync(boff, bo.data(), 8 * bo.size(), hipMemcpyHostToDevice, s));
-        g->b_src = dmalloc<int32_t>((size_t)Ep);
-        g->b_dst = dmalloc<uint16_t>((size_t)Ep);
+        // whole sources and destination offsets first, packed below
+        int32_t *bs = dmalloc<int32_t>((size_t)Ep);
+        uint16_t *bd = dmalloc<uint16_t>((size_t)Ep);
+        uint8_t *wflag = dmalloc<uint8_t>((size_t)nch);
+        int32_t *widx = dmalloc<int32_t>((size_t)nch);
+        struct F4 {
+            void *p[4];
+            ~F4()
+            {
+                for (void *x : p) dfree(x);
+            }
+        } f4{{bs, bd, wflag, widx}};
         g->b_w = dmalloc<uint32_t>((size_t)Ep);
         g->n_btile = (int32_t)tiles.size();
         g->b_edges = Ep;
@@ -965,12 +1034,27 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         SSSP_HIP(hipMemcpyAsync(g->b_tile, tiles.data(), sizeof(BlkTile) * tiles.size(),
                                 hipMemcpyHostToDevice, s));
         k_blk_gather<<<grid_for(E), kBlock, 0, s>>>(bkeys, bperm, E, bptr, boff, g->el_src,
-                                                    g->el_dst, g->el_w, g->b_src, g->b_dst,
-                                                    g->b_w);
+                                                    g->el_dst, g->el_w, bs, bd, g->b_w);
         SSSP_LAUNCH_CHECK("k_blk_gather");
-        k_blk_pad<<<grid_for((int64_t)nblk * 4), kBlock, 0, s>>>(bptr, boff, nblk, g->b_src,
-                                                                 g->b_dst, g->b_w);
+        k_blk_pad<<<grid_for((int64_t)nblk * kBlkChunk), kBlock, 0, s>>>(bptr, boff, nblk, bs, bd,
+                                                                         g->b_w);
         SSSP_LAUNCH_CHECK("k_blk_pad");
+        const unsigned chunk_grid = (unsigned)((nch * 64 + kBlock - 1) / kBlock);
+        k_blk_chunk_wide<<<chunk_grid, kBlock, 0, s>>>(bs, nch, wflag);
+        SSSP_LAUNCH_CHECK("k_blk_chunk_wide");
+        std::vector<uint8_t> wf((size_t)nch);
+        SSSP_HIP(hipMemcpyAsync(wf.data(), wflag, (size_t)nch, hipMemcpyDeviceToHost, s));
+        SSSP_HIP(hipStreamSynchronize(s));
+        std::vector<int32_t> wi((size_t)nch);
+        int64_t nw = 0;
+        for (int64_t c = 0; c < nch; c++) wi[(size_t)c] = wf[(size_t)c] ? (int32_t)nw++ : -1;
+        SSSP_HIP(hipMemcpyAsync(widx, wi.data(), 4 * (size_t)nch, hipMemcpyHostToDevice, s));
+        g->b_pk = dmalloc<uint32_t>((size_t)Ep);
+        g->b_cbase = dmalloc<int32_t>((size_t)nch);
+        g->b_wide = nw ? dmalloc<int32_t>((size_t)(nw * kBlkChunk)) : nullptr;
+        g->b_nwide = nw;
+        k_blk_pack<<<chunk_grid, kBlock, 0, s>>>(bs, bd, widx, nch, g->b_pk, g->b_cbase, g->b_wide);
+        SSSP_LAUNCH_CHECK("k_blk_pack");
         SSSP_HIP(hipStreamSynchronize(s));
     }
 

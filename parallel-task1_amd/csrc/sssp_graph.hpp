@@ -60,13 +60,18 @@ struct Ctrl {
 };
 
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
-// block's run padded to a multiple of 4 edges with weight-0 self loops on the
-// block's first vertex; destinations are stored as 16-bit offsets in their
-// block. A tile is a run of at most kBlkTileEdges edges of one block (the
-// cap doubles while the graph holds kBlkTileCount tiles' worth of edges or
-// more); `split` = the block has more than one tile.
+// block's run padded to whole chunks of kBlkChunk slots with copies of its
+// last edge (relaxing an edge twice changes nothing). An edge is 8 bytes: a
+// 32-bit word holding its destination's offset in the block (low kBlkBits
+// bits) and its source as the delta to the previous edge of the chunk, and
+// its weight; each chunk keeps its first source whole. A tile is a run of
+// at most kBlkTileEdges slots of one block, whole chunks (the cap doubles
+// while the graph holds kBlkTileCount tiles' worth of edges or more);
+// `split` = the block has more than one tile.
 constexpr int kBlkBits = 14;
-static_assert(kBlkBits <= 16, "block offsets are stored in 16 bits");
+static_assert(kBlkBits <= 16, "block offsets fit the low half of a packed edge");
+constexpr int kBlkDeltaBits = 32 - kBlkBits;  // source delta bits of a packed edge
+constexpr int64_t kBlkChunk = 256;  // edge slots per chunk: 4 per lane of one wave
 constexpr int64_t kBlkTileEdges = (int64_t)1 << 17;
 constexpr int64_t kBlkTileCount = 4096;
 struct BlkTile {
@@ -125,9 +130,16 @@ struct sssp_graph {
     uint32_t *x_w = nullptr;
     int64_t *xgoff = nullptr;
     std::vector<int64_t> xgoff_host;
-    // destination-blocked edge list (dense all-edge rounds)
-    int32_t *b_src = nullptr;
-    uint16_t *b_dst = nullptr;  // destination - block base
+    // destination-blocked edge list (dense all-edge rounds and the blocked
+    // plain pass), in chunks of kBlkChunk edge slots: b_pk[e] = source delta
+    // (to the previous edge of the chunk) << kBlkBits | (destination - block
+    // base); b_cbase[c] = source of chunk c's first edge, or -(1 + i) for a
+    // chunk whose sources are kept whole in b_wide[i * kBlkChunk ...] (a gap
+    // of 2^kBlkDeltaBits or more between two of its sources)
+    uint32_t *b_pk = nullptr;
+    int32_t *b_cbase = nullptr;
+    int32_t *b_wide = nullptr;
+    int64_t b_nwide = 0;  // chunks with whole sources
     uint32_t *b_w = nullptr;
     sssp::BlkTile *b_tile = nullptr;
     int32_t n_btile = 0;
@@ -336,8 +348,8 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
                    bool build_csc);
 // the multi-source records and tiles (m_rec, m_tile), built once per graph
 void build_multi_layout(sssp_graph *g);
-// rebuilds the destination-blocked list's tile table (cap: edges per tile, a
-// multiple of 4; order: 0 smallest first, 1 by block, 2 largest first, 3 by
+// rebuilds the destination-blocked list's tile table (cap: edge slots per
+// tile, a multiple of kBlkChunk; order: 0 smallest first, 1 by block, 2 largest first, 3 by
 // descending block)
 void retile_blocked(sssp_graph *g, int64_t cap, int order);
 void device_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *blk_a,

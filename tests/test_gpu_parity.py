@@ -350,6 +350,38 @@ def test_plain_kernels_bit_exact(gpu, oracle, kernel):
 
 
 @pytest.mark.gpu
+def test_blocked_list_wide_chunks(gpu, oracle):
+    """The blocked list packs each source as a delta to the previous edge of
+    its 256-edge chunk; a chunk with a gap of 2^18 or more keeps its sources
+    whole. A sparse graph on 2^20 vertices (few in-edges per block, sources
+    far apart) has such chunks next to packed ones; the plain blocked pass,
+    the dense rounds and BFS stay exact on both."""
+    V = 1 << 20
+    rng = np.random.default_rng(5)
+    n = 1500
+    src = rng.integers(0, V, n).astype(np.int32)
+    dst = rng.integers(0, V, n).astype(np.int32)
+    hop = np.sort(rng.choice(V, 400, replace=False)).astype(np.int32)  # a long path
+    src = np.concatenate([src, hop[:-1], np.full(300, 7, np.int32)])
+    dst = np.concatenate([dst, hop[1:], rng.integers(0, 1 << 14, 300).astype(np.int32)])
+    w = rng.random(len(src), dtype=np.float32)
+    with S.DeviceGraph(src, dst, w, V, relabel=False) as g:
+        nw = g.get_param("blk_wide_chunks")
+        print(f"wide chunks: {nw} of {g.get_param('blk_tiles')} tiles", flush=True)
+        assert nw > 0
+        g.set_param("plain_kernel", 2)
+        g.set_param("plain_below_edges", 0)
+        g.set_param("dense_all_frac", 0.0)  # every all-edge round dense
+        for s in (int(hop[0]), 7, int(src[3])):
+            ref = oracle.dijkstra(src, dst, w, V, s)
+            for sched in ("plain", "all", "heuristic", "learned"):
+                d, _ = g.solve(s, sched)
+                assert np.array_equal(bits(d), bits(ref)), (s, sched)
+            _, lev, _ = g.bfs(s, "all")
+            assert np.array_equal(lev, oracle.bfs(src, dst, V, s)[1]), s
+
+
+@pytest.mark.gpu
 def test_plain_kernel_choice(gpu):
     """plain_kernel 0 picks the blocked pass once the blocked list has 64
     tiles or more (here forced by a small tile cap), else the edge-list
