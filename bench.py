@@ -470,6 +470,7 @@ def main():
         "all_dense_gteps": round(results["all_dense"]["gteps"], 3),
         "heuristic_gteps": round(results["heuristic"]["gteps"], 3),
         "speedup_vs_plain": round(learned["gteps"] / results["plain"]["gteps"], 3),
+        "speedup_vs_plain_edge_list": round(learned["gteps"] / results["plain_edge_list"]["gteps"], 3),
         "learned_vs_heuristic": round(learned["gteps"] / results["heuristic"]["gteps"], 3),
         "certified_identical": certified,
         "scheduler_network": "loaded" if mlp else "absent",

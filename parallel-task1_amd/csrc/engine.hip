@@ -773,10 +773,13 @@ __global__ __launch_bounds__(kBlock) void k_init_sources(Ctrl *c, uint32_t *dist
 // Zeroes the slot the coming round fills and samples the unsettled fraction
 // of the frontier it consumes (d > minF, minF = min distance over the frontier
 // and, pile >= 0, over the far pile).
-__global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r, int pile,
-                                                        const int32_t *sample_v, int32_t n_sample,
-                                                        const uint32_t *dist,
-                                                        const int64_t *col_ptr, uint32_t inf)
+// Round bookkeeping: resets the next round's slot and samples the unsettled
+// (distance above the frontier minimum) and unreached fractions of a fixed
+// vertex sample, with the in-degree of the unsettled ones (pull estimate).
+__device__ __forceinline__ void round_sample(Ctrl *c, int32_t r, int pile,
+                                             const int32_t *sample_v, int32_t n_sample,
+                                             const uint32_t *dist, const int64_t *col_ptr,
+                                             uint32_t inf)
 {
     RoundSlot *cur = &c->slot[r & 1];
     RoundSlot *nxt = &c->slot[(r + 1) & 1];
@@ -807,6 +810,14 @@ __global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r, int 
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r, int pile,
+                                                        const int32_t *sample_v, int32_t n_sample,
+                                                        const uint32_t *dist,
+                                                        const int64_t *col_ptr, uint32_t inf)
+{
+    round_sample(c, r, pile, sample_v, n_sample, dist, col_ptr, inf);
+}
+
 struct DecideArgs {
     int32_t r;
     int32_t sched;
@@ -821,9 +832,12 @@ struct DecideArgs {
     int32_t seq;       // written to mirror->dec_seq once the mirror holds this round
 };
 
-// One wave: features, the hand-written rule and (learned schedule) the MLP.
-__global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
+// Features, the hand-written rule and (learned schedule) the MLP, computed
+// by threads 0-63 of one workgroup (every thread of it calls this: it holds
+// workgroup barriers); writes the decision and the host mirror.
+__device__ __forceinline__ void decide_round(Ctrl *c, const DecideArgs &d)
 {
+    static_assert(kHid == 64, "one hidden unit per thread of the first wave");
     __shared__ float s_f[kIn];
     __shared__ float s_h1[kHid];
     __shared__ float s_h2[kHid];
@@ -859,16 +873,20 @@ __global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
     int mode = hmode, chunk = hchunk, by_mlp = 0;
     if (d.mlp && d.sched == SSSP_SCHED_LEARNED) {
         const float *P = d.mlp;
-        float acc = P[kOffB1 + j];
-        for (int i = 0; i < kIn; i++) acc += P[kOffW1 + j * kIn + i] * s_f[i];
-        s_h1[j] = acc > 0.f ? acc : 0.f;
+        if (j < kHid) {
+            float acc = P[kOffB1 + j];
+            for (int i = 0; i < kIn; i++) acc += P[kOffW1 + j * kIn + i] * s_f[i];
+            s_h1[j] = acc > 0.f ? acc : 0.f;
+        }
         __syncthreads();
-        acc = P[kOffB2 + j];
-        for (int i = 0; i < kHid; i++) acc += P[kOffW2 + j * kHid + i] * s_h1[i];
-        s_h2[j] = acc > 0.f ? acc : 0.f;
+        if (j < kHid) {
+            float acc = P[kOffB2 + j];
+            for (int i = 0; i < kHid; i++) acc += P[kOffW2 + j * kHid + i] * s_h1[i];
+            s_h2[j] = acc > 0.f ? acc : 0.f;
+        }
         __syncthreads();
         if (j < kOut) {
-            acc = P[kOffB3 + j];
+            float acc = P[kOffB3 + j];
             for (int i = 0; i < kHid; i++) acc += P[kOffW3 + j * kHid + i] * s_h2[i];
             s_o[j] = acc;
         }
@@ -922,6 +940,27 @@ __global__ __launch_bounds__(64) void k_decide(Ctrl *c, DecideArgs d)
         __threadfence_system();
         __hip_atomic_store(&m->dec_seq, d.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+}
+
+// Round bookkeeping and decision in one launch: every workgroup samples
+// (round_sample), then counts itself done; the last one to finish -- which
+// sees every other workgroup's sample counts through the fences around the
+// counter -- resets the counter and decides.
+__global__ __launch_bounds__(kBlock) void k_round_open(Ctrl *c, int32_t r, int pile,
+                                                       const int32_t *sample_v, int32_t n_sample,
+                                                       const uint32_t *dist, const int64_t *col_ptr,
+                                                       uint32_t inf, DecideArgs d)
+{
+    __shared__ int s_last;
+    round_sample(c, r, pile, sample_v, n_sample, dist, col_ptr, inf);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&c->open_done, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;  // workgroup-uniform
+    __threadfence();
+    if (threadIdx.x == 0) c->open_done = 0;
+    decide_round(c, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -1473,8 +1512,7 @@ void lb_prepare(sssp_graph *g, const int32_t *q, int32_t n, const int32_t *dev_n
     k_lb_reduce<<<(unsigned)nb, kBlock, 0, s>>>(q, n, dev_n, g->row_ptr, g->dist, rule, g->blk_a,
                                                 g->blk_b);
     SSSP_LAUNCH_CHECK("k_lb_reduce");
-    scan_blocks_inplace(g->blk_a, nb, s);
-    scan_blocks_inplace(g->blk_b, nb, s);
+    scan_blocks_pair_inplace(g->blk_a, g->blk_b, nb, s);
     LbWriteArgs w{q, n, dev_n, g->row_ptr, g->dist, rule, g->blk_a, g->blk_b, nb, g->lb_v,
                   g->lb_start, g->lb_scan, g->lb_tile, TE, g->ctrl, g->farst, g->pile[nxt], nxt};
     k_lb_write<<<(unsigned)nb, kBlock, 0, s>>>(w);
@@ -1712,8 +1750,8 @@ void wait_decision(sssp_graph *g, int32_t seq)
     SSSP_HIP(hipStreamSynchronize(g->stream));
 }
 
-// One round r: round-begin kernel, decision, host read-back of the frontier
-// counters, then the round itself. Returns false, having run nothing, when
+// One round r: bookkeeping and decision (k_round_open, one launch), host
+// read-back of the frontier counters, then the round itself. Returns false, having run nothing, when
 // the solve is over (no frontier, no pile, fixpoint check done or off).
 // force_mode >= 0 replaces the decision (not the fixpoint check).
 template <class WT>
@@ -1723,10 +1761,6 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     hipStream_t s = g->stream;
     const bool learned = (sched == SSSP_SCHED_LEARNED);
     const int ns = g->n_sample;
-    k_round_begin<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v, ns,
-                                                      g->dist, g->has_csc ? g->col_ptr : nullptr,
-                                                      WT::INF);
-    SSSP_LAUNCH_CHECK("k_round_begin");
     DecideArgs da{};
     da.r = r;
     da.sched = sched;
@@ -1743,8 +1777,10 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     da.pile = g->pile_cur;
     da.mirror = g->ctrl_host_dev;
     da.seq = ++g->dec_seq;
-    k_decide<<<1, 64, 0, s>>>(g->ctrl, da);
-    SSSP_LAUNCH_CHECK("k_decide");
+    k_round_open<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v, ns,
+                                                     g->dist, g->has_csc ? g->col_ptr : nullptr,
+                                                     WT::INF, da);
+    SSSP_LAUNCH_CHECK("k_round_open");
     wait_decision(g, da.seq);
     const RoundSlot cur = g->ctrl_host->slot[r & 1];
     const int32_t piled = g->ctrl_host->pile_n[g->pile_cur];

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int64_t *in, int64
 }
 
 // single block: exclusive scan of blk[0..nb) in place, blk[nb] = total
-__global__ __launch_bounds__(kBlock) void k_scan_blocks(int64_t *blk, int64_t nb)
+__device__ __forceinline__ void scan_blocks_body(int64_t *blk, int64_t nb)
 {
     __shared__ int64_t s_warp[kBlock / 64];
     int64_t carry = 0;
@@ -52,6 +52,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_blocks(int64_t *blk, int64_t nb
         carry += tot;
     }
     if (threadIdx.x == 0) blk[nb] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_blocks(int64_t *blk, int64_t nb)
+{
+    scan_blocks_body(blk, nb);
+}
+
+// two arrays of nb + 1 values in one launch: workgroup 0 scans a, 1 scans b
+__global__ __launch_bounds__(kBlock) void k_scan_blocks_pair(int64_t *a, int64_t *b, int64_t nb)
+{
+    scan_blocks_body(blockIdx.x == 0 ? a : b, nb);
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_down(const int64_t *in, int64_t *out, int64_t n,
@@ -603,6 +614,12 @@ void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s)
 {
     k_scan_blocks<<<1, kBlock, 0, s>>>(blk, nb);
     SSSP_LAUNCH_CHECK("k_scan_blocks");
+}
+
+void scan_blocks_pair_inplace(int64_t *a, int64_t *b, int64_t nb, hipStream_t s)
+{
+    k_scan_blocks_pair<<<2, kBlock, 0, s>>>(a, b, nb);
+    SSSP_LAUNCH_CHECK("k_scan_blocks_pair");
 }
 
 int64_t scan_blocks_needed(int64_t n) { return (n + kScanTile - 1) / kScanTile + 2; }

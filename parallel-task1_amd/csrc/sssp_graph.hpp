@@ -57,6 +57,7 @@ struct Ctrl {
     int32_t lb_n;             // queue entries plus pile entries moved onto the queue
     int32_t dec_seq;          // host mirror: written last by the decision kernel (round
                               // sequence number), after the fields above
+    int32_t open_done;        // workgroups of k_round_open done sampling (the last one decides)
 };
 
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
@@ -341,6 +342,8 @@ void rmat_device(int scale, int ef, int a, int b, int c, int d, uint32_t base_se
 void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int wtype, int64_t E,
                     int32_t V, hipStream_t s, int64_t *bad_buf = nullptr);
 void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s);
+// the same for two arrays in one launch
+void scan_blocks_pair_inplace(int64_t *a, int64_t *b, int64_t nb, hipStream_t s);
 // packed 12-byte binary edge-list records (device) -> src, dst, w (device)
 void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, uint32_t *w,
                    hipStream_t s);
