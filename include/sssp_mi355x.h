@@ -184,8 +184,9 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
  * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
  * "batch_capture", "decide_spin", "blk_tile_cap", "blk_tile_order",
- * "plain_kernel", "trace_events"; "cu_count", "collect_rounds",
- * "multi_passes", "blk_tiles" and "plain_blocked" read-only).
+ * "blk_range_bits", "plain_kernel", "trace_events"; "cu_count",
+ * "collect_rounds", "multi_passes", "blk_tiles", "blk_wide_chunks" and
+ * "plain_blocked" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
  * 2^14 destinations with twice the mean in-edges) and dense_unreached
  * (default 0.5): an all-edge round whose frontier has at least
@@ -225,10 +226,16 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * one host check per run of 32 passes; 0 queues the passes from the host, four
  * per check. "multi_passes" (read-only) = passes the last policy-3 batch ran.
  * blk_tile_cap / blk_tile_order rebuild the tile table of the dense rounds'
- * destination-blocked list: at most blk_tile_cap edges per tile (a multiple
- * of 4; default 2^17, doubled while 4096 tiles of the cap fit in E) and the
- * dispatch order 0 = smallest tiles first (default), 1 = by block, 2 = largest
- * first, 3 = by descending block. decide_spin (default 1): the host polls the
+ * destination-blocked list: at most blk_tile_cap edge slots per tile (a
+ * multiple of 256; default 2^17, doubled while 4096 tiles of the cap fit in
+ * E) and the dispatch order 0 = smallest tiles first (default), 1 = by block,
+ * 2 = largest first, 3 = by descending block. blk_range_bits > 0 (18-31)
+ * also cuts every block's tiles where their sources cross multiples of
+ * 2^blk_range_bits and dispatches the tiles range by range, so that the
+ * source gathers of the workgroups in flight stay in one range; the default
+ * is 0, or, for a graph of 2^22 vertices or more without hub blocks, the size
+ * that makes 8 ranges. "blk_wide_chunks" (read-only) = chunks of 256 edges
+ * whose sources are stored whole (a gap of 2^18 or more between two of them). decide_spin (default 1): the host polls the
  * mapped control block for each round's decision instead of synchronizing
  * the stream.
  * Distances do not depend on any knob. */

@@ -587,11 +587,15 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
             SSSP_REQUIRE(value >= (double)kBlkChunk && value <= 2147483648.0 &&
                              std::fmod(value, (double)kBlkChunk) == 0,
                          "blk_tile_cap must be a multiple of 256 in [256, 2^31]");
-            retile_blocked(g, (int64_t)value, g->blk_tile_order);
+            retile_blocked(g, (int64_t)value, g->blk_tile_order, g->blk_range_bits);
         } else if (n == "blk_tile_order") {
             SSSP_REQUIRE(value == 0 || value == 1 || value == 2 || value == 3,
                          "blk_tile_order must be 0, 1, 2 or 3");
-            retile_blocked(g, g->blk_tile_cap, (int)value);
+            retile_blocked(g, g->blk_tile_cap, (int)value, g->blk_range_bits);
+        } else if (n == "blk_range_bits") {
+            SSSP_REQUIRE(value == 0 || (value >= kBlkRangeFineBits && value <= 31),
+                         "blk_range_bits must be 0 or in [18, 31]");
+            retile_blocked(g, g->blk_tile_cap, g->blk_tile_order, (int)value);
         } else if (n == "decide_spin") {
             SSSP_REQUIRE(value == 0 || value == 1, "decide_spin must be 0 or 1");
             g->decide_spin = (int)value;
@@ -643,6 +647,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "blk_tile_order") *value = g->blk_tile_order;
         else if (n == "blk_tiles") *value = g->n_btile;
         else if (n == "blk_wide_chunks") *value = (double)g->b_nwide;
+        else if (n == "blk_range_bits") *value = g->blk_range_bits;
         else if (n == "plain_kernel") *value = g->plain_kernel;
         else if (n == "plain_blocked") *value = plain_uses_blocked(g) ? 1 : 0;
         else if (n == "multi_passes") *value = g->multi_passes;

@@ -432,6 +432,34 @@ __global__ __launch_bounds__(kBlock) void k_blk_pack(const int32_t *bs, const ui
     if (__lane_id() == 0) cbase[c] = wi < 0 ? s[0] : -(1 + wi);
 }
 
+// off[b * (nfine + 1) + k] = the first slot of block b whose source is at
+// least k << fine_bits (sources ascend within a block's padded run), aligned
+// down to its chunk; k = nfine gives the end of the run.
+__global__ __launch_bounds__(kBlock) void k_blk_range_offsets(const int32_t *bs, const int64_t *boff,
+                                                              int32_t nblk, int fine_bits,
+                                                              int32_t nfine, int64_t *off)
+{
+    const int64_t n = (int64_t)nblk * (nfine + 1);
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const int32_t b = (int32_t)(i / (nfine + 1));
+        const int32_t k = (int32_t)(i % (nfine + 1));
+        const int64_t b0 = boff[b], b1 = boff[b + 1];
+        int64_t pos = b1;
+        if (k < nfine) {
+            const int64_t key = (int64_t)k << fine_bits;
+            int64_t lo = b0, hi = b1;  // first position with bs >= key
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((int64_t)bs[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            pos = b0 + (lo - b0) / kBlkChunk * kBlkChunk;
+        }
+        off[i] = pos;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_m_keys(const int32_t *dst, int64_t E, uint32_t *keys)
 {
     for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < E;
@@ -754,35 +782,69 @@ void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int w
 }
 
 // The tiles of the destination-blocked list: runs of at most `cap` edge slots
-// (a multiple of kBlkChunk) of one block, `split` when a block has several; order 0 =
-// smallest tiles first (stable by block), 1 = by block, 2 = largest first,
-// 3 = by descending block. bo: padded block offsets, nblk + 1 values.
-std::vector<BlkTile> make_blk_tiles(const std::vector<int64_t> &bo, int64_t cap, int order)
+// (a multiple of kBlkChunk) of one block, `split` when a block has several.
+// Without source ranges (R.stride == 0) a tile is a run of its block's slots;
+// order 0 = smallest tiles first (stable by block), 1 = by block, 2 = largest
+// first, 3 = by descending block. With source ranges each block's run is cut
+// at the (chunk-aligned) slots where its sources reach multiples of the range
+// size, and the tiles are dispatched range by range (by block within one):
+// the workgroups running at one time then gather d[src] from one range of
+// the distance array, which an XCD's L2 holds. bo: padded block offsets,
+// nblk + 1 values.
+std::vector<BlkTile> make_blk_tiles(const std::vector<int64_t> &bo, int64_t cap, int order,
+                                    const BlkRanges &R)
 {
-    std::vector<BlkTile> tiles;
+    struct T {
+        BlkTile t;
+        int32_t range;
+    };
+    std::vector<T> tiles;
     const int32_t nblk = (int32_t)bo.size() - 1;
+    const bool ranged = R.stride > 0 && !R.off.empty();
+    const int32_t nr = ranged ? (R.n_fine + R.stride - 1) / R.stride : 1;
     for (int32_t b = 0; b < nblk; b++) {
-        const int64_t c = bo[(size_t)b + 1] - bo[(size_t)b];
-        const int64_t nt = (c + cap - 1) / cap;
-        for (int64_t k = 0; k < nt; k++) {
-            const int64_t e0 = bo[(size_t)b] + k * cap;
-            tiles.push_back(BlkTile{b, nt > 1 ? 1 : 0, e0, std::min(e0 + cap, bo[(size_t)b + 1])});
+        const size_t first = tiles.size();
+        for (int32_t r = 0; r < nr; r++) {
+            int64_t lo = bo[(size_t)b], hi = bo[(size_t)b + 1];
+            if (ranged) {
+                const size_t row = (size_t)b * (size_t)(R.n_fine + 1);
+                lo = R.off[row + (size_t)std::min(r * R.stride, R.n_fine)];
+                hi = R.off[row + (size_t)std::min((r + 1) * R.stride, R.n_fine)];
+            }
+            for (int64_t e0 = lo; e0 < hi; e0 += cap)
+                tiles.push_back(T{BlkTile{b, 0, e0, std::min(e0 + cap, hi)}, r});
         }
+        if (tiles.size() - first > 1)
+            for (size_t i = first; i < tiles.size(); i++) tiles[i].t.split = 1;
     }
-    if (order == 0 || order == 2)
-        std::stable_sort(tiles.begin(), tiles.end(), [order](const BlkTile &a, const BlkTile &b) {
-            return order == 0 ? a.e1 - a.e0 < b.e1 - b.e0 : a.e1 - a.e0 > b.e1 - b.e0;
+    if (ranged)
+        std::stable_sort(tiles.begin(), tiles.end(),
+                         [](const T &a, const T &b) { return a.range < b.range; });
+    else if (order == 0 || order == 2)
+        std::stable_sort(tiles.begin(), tiles.end(), [order](const T &a, const T &b) {
+            const int64_t la = a.t.e1 - a.t.e0, lb = b.t.e1 - b.t.e0;
+            return order == 0 ? la < lb : la > lb;
         });
     else if (order == 3)
         std::stable_sort(tiles.begin(), tiles.end(),
-                         [](const BlkTile &a, const BlkTile &b) { return a.blk > b.blk; });
-    return tiles;
+                         [](const T &a, const T &b) { return a.t.blk > b.t.blk; });
+    std::vector<BlkTile> out;
+    out.reserve(tiles.size());
+    for (const T &x : tiles) out.push_back(x.t);
+    return out;
 }
 
-void retile_blocked(sssp_graph *g, int64_t cap, int order)
+void retile_blocked(sssp_graph *g, int64_t cap, int order, int range_bits)
 {
     if (g->b_off_host.empty()) return;
-    std::vector<BlkTile> tiles = make_blk_tiles(g->b_off_host, cap, order);
+    BlkRanges R = g->b_ranges;
+    R.stride = 0;
+    if (range_bits > 0) {
+        if (R.off.empty() || range_bits < R.fine_bits || range_bits > 31)
+            throw Error{SSSP_ERR_ARG, "blk_range_bits: source ranges unavailable for this size"};
+        R.stride = 1 << (range_bits - R.fine_bits);
+    }
+    std::vector<BlkTile> tiles = make_blk_tiles(g->b_off_host, cap, order, R);
     BlkTile *td = dmalloc<BlkTile>(std::max<size_t>(tiles.size(), 1));
     SSSP_HIP(hipMemcpyAsync(td, tiles.data(), sizeof(BlkTile) * tiles.size(), hipMemcpyHostToDevice,
                             g->stream));
@@ -792,6 +854,7 @@ void retile_blocked(sssp_graph *g, int64_t cap, int order)
     g->n_btile = (int32_t)tiles.size();
     g->blk_tile_cap = cap;
     g->blk_tile_order = order;
+    g->blk_range_bits = range_bits;
 }
 
 void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, const uint32_t *d_w,
@@ -1012,7 +1075,8 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         int64_t max_blk = 0;
         for (int32_t b = 0; b < nblk; b++)
             max_blk = std::max(max_blk, bp[(size_t)b + 1] - bp[(size_t)b]);
-        if ((double)max_blk * nblk < 2.0 * (double)E) g->dense_all_frac = 0.4;
+        const bool no_hubs = (double)max_blk * nblk < 2.0 * (double)E;
+        if (no_hubs) g->dense_all_frac = 0.4;
         for (int32_t b = 0; b < nblk; b++)
             bo[(size_t)b + 1] = bo[(size_t)b] +
                                 (bp[(size_t)b + 1] - bp[(size_t)b] + kBlkChunk - 1) / kBlkChunk * kBlkChunk;
@@ -1026,7 +1090,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         int64_t tcap = kBlkTileEdges;
         while (tcap * kBlkTileCount <= E) tcap <<= 1;
         g->b_off_host = bo;
-        std::vector<BlkTile> tiles = make_blk_tiles(bo, tcap, 0);
+        std::vector<BlkTile> tiles = make_blk_tiles(bo, tcap, 0, BlkRanges{});
         g->blk_tile_cap = tcap;
         g->blk_tile_order = 0;
         const int64_t Ep = bo[(size_t)nblk];
@@ -1072,7 +1136,40 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         g->b_nwide = nw;
         k_blk_pack<<<chunk_grid, kBlock, 0, s>>>(bs, bd, widx, nch, g->b_pk, g->b_cbase, g->b_wide);
         SSSP_LAUNCH_CHECK("k_blk_pack");
+        // source-range cuts of every block at 2^kBlkRangeFineBits granularity
+        // (for tiles dispatched range by range), when the table is small
+        const int64_t nfine = ((int64_t)V + (1 << kBlkRangeFineBits) - 1) >> kBlkRangeFineBits;
+        if ((int64_t)nblk * (nfine + 1) <= kBlkRangeMaxEntries) {
+            const int64_t n_off = (int64_t)nblk * (nfine + 1);
+            int64_t *roff = dmalloc<int64_t>((size_t)n_off);
+            struct F1 {
+                void *p;
+                ~F1() { dfree(p); }
+            } f1{roff};
+            k_blk_range_offsets<<<grid_for(n_off), kBlock, 0, s>>>(bs, boff, nblk, kBlkRangeFineBits,
+                                                                  (int32_t)nfine, roff);
+            SSSP_LAUNCH_CHECK("k_blk_range_offsets");
+            g->b_ranges.off.resize((size_t)n_off);
+            SSSP_HIP(hipMemcpyAsync(g->b_ranges.off.data(), roff, 8 * (size_t)n_off,
+                                    hipMemcpyDeviceToHost, s));
+            g->b_ranges.fine_bits = kBlkRangeFineBits;
+            g->b_ranges.n_fine = (int32_t)nfine;
+        }
         SSSP_HIP(hipStreamSynchronize(s));
+        // Without hubs the d[src] gathers of a tile spread over the whole
+        // distance array (degree order cannot gather them), so a graph whose
+        // distances outgrow the L2 several times over runs its tiles by source
+        // range, 8 ranges (uniform random graphs, ms learned / plain / dense
+        // round: 2^24 x 16 31.7 / 71.7 / 2.96 by block, 23.6 / 36.5 / 1.57 in
+        // 8 ranges, 26.4 / 37.1 / 1.83 in 16, 23.9 / 41.6 / 1.75 in 4; 2^22 x 16
+        // 7.75 / 11.1 / 0.38 by block, 7.09 / 7.93 / 0.38 in 8 ranges). Graphs
+        // with hubs lose by it (R-MAT 25 x 16: 17.2 / 24.7 ms by block, 18.5 /
+        // 26.4 in 8 ranges).
+        if (no_hubs && V >= (1 << kBlkRangeMinVertexBits) && !g->b_ranges.off.empty()) {
+            int lg = 0;
+            while (((int64_t)1 << lg) < (int64_t)V) lg++;
+            retile_blocked(g, tcap, 0, std::max(kBlkRangeFineBits, lg - 3));
+        }
     }
 
     // ---- CSC ----

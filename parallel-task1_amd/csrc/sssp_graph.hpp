@@ -80,6 +80,19 @@ struct BlkTile {
     int32_t split;
     int64_t e0, e1;
 };
+// Source-range cuts of the blocked list's block runs (host copy): off[b *
+// (n_fine + 1) + k] = first slot of block b whose source is at least
+// k << fine_bits, aligned down to its chunk (k = n_fine: the run's end).
+// Tiles cut at every stride-th cut (stride 0: no cuts).
+constexpr int kBlkRangeFineBits = 18;
+constexpr int64_t kBlkRangeMaxEntries = (int64_t)1 << 22;
+constexpr int kBlkRangeMinVertexBits = 22;  // source ranges by default from 2^22 vertices
+struct BlkRanges {
+    std::vector<int64_t> off;
+    int32_t fine_bits = 0;
+    int32_t n_fine = 0;
+    int32_t stride = 0;
+};
 
 // Multi-source batches on one graph: edges as 16-byte records sorted by
 // (dst >> kMBits, src) -- 16-vertex destination blocks, so a 256-source block
@@ -147,6 +160,8 @@ struct sssp_graph {
     std::vector<int64_t> b_off_host;  // padded block offsets (tiles are rebuilt from them)
     int64_t blk_tile_cap = 0;         // edges per tile at most
     int blk_tile_order = 0;           // dispatch order of the tiles (make_blk_tiles)
+    sssp::BlkRanges b_ranges;         // source-range cuts of the block runs (may be empty)
+    int blk_range_bits = 0;           // tiles cut by source ranges of 2^bits vertices (0: no)
     int64_t b_edges = 0;  // edges in the blocked list, padding included
     // multi-source batches (built on first use): records, tiles, and the
     // vertex-major distances of up to 256 sources (D[v * n + s])
@@ -354,7 +369,7 @@ void build_multi_layout(sssp_graph *g);
 // rebuilds the destination-blocked list's tile table (cap: edge slots per
 // tile, a multiple of kBlkChunk; order: 0 smallest first, 1 by block, 2 largest first, 3 by
 // descending block)
-void retile_blocked(sssp_graph *g, int64_t cap, int order);
+void retile_blocked(sssp_graph *g, int64_t cap, int order, int range_bits = 0);
 void device_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *blk_a,
                                int64_t *blk_b, int64_t n_blk_cap, hipStream_t s);
 int64_t scan_blocks_needed(int64_t n);

@@ -38,6 +38,9 @@ def test_small_graphs_every_schedule(gpu, oracle, name, V, src, dst, w):
         for sched in SCHEDS:
             d, st = g.solve(0, sched)
             _check_solution(g, oracle, src, dst, w, V, 0, d, sched)
+        g.set_param("plain_kernel", 2)  # the plain pass over the blocked list too
+        d, st = g.solve(0, "plain")
+        _check_solution(g, oracle, src, dst, w, V, 0, d, "plain (blocked)")
     if name == "chain":
         assert np.array_equal(d, (np.arange(10) * np.float32(0.5)).astype(np.float32))
     if name == "disconnected":
@@ -379,6 +382,47 @@ def test_blocked_list_wide_chunks(gpu, oracle):
                 assert np.array_equal(bits(d), bits(ref)), (s, sched)
             _, lev, _ = g.bfs(s, "all")
             assert np.array_equal(lev, oracle.bfs(src, dst, V, s)[1]), s
+
+
+@pytest.mark.gpu
+def test_blocked_tiles_by_source_range(gpu, oracle):
+    """Tiles cut by source range and dispatched range by range (blk_range_bits):
+    the default for a graph without hubs from 2^22 vertices (8 ranges), off for
+    R-MAT; any range size keeps the dense rounds and the plain blocked pass on
+    the oracle's fixpoint, f32 and u32."""
+    V = 1 << 20
+    src, dst, w = S.er(V, 4, seed=6, weight_mode=0)
+    wu = (np.arange(len(src)) % 255 + 1).astype(np.uint32)
+    with S.DeviceGraph(src, dst, w, V) as g, S.DeviceGraph(src, dst, wu, V) as gu:
+        assert g.get_param("blk_range_bits") == 0  # below 2^22 vertices
+        ref = oracle.dijkstra(src, dst, w, V, 5)
+        refu = oracle.dijkstra(src, dst, wu, V, 5)
+        for h in (g, gu):
+            h.set_param("plain_below_edges", 0)
+            h.set_param("dense_all_frac", 0.0)  # every all-edge round dense
+            h.set_param("plain_kernel", 2)
+        for rb in (18, 19, 20, 0):
+            print(f"source ranges of 2^{rb}", flush=True)
+            for h in (g, gu):
+                h.set_param("blk_range_bits", rb)
+            assert g.get_param("blk_range_bits") == rb
+            for sched in ("all", "plain", "heuristic"):
+                d, _ = g.solve(5, sched)
+                assert np.array_equal(bits(d), bits(ref)), (rb, sched)
+            du, _ = gu.solve(5, "all")
+            assert np.array_equal(du, refu), rb
+        for bad in (5, 17, 32):
+            with pytest.raises(S.SSSPError):
+                g.set_param("blk_range_bits", bad)
+    V = 1 << 22
+    src, dst, w = S.er(V, 2, seed=7, weight_mode=0)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        assert g.get_param("blk_range_bits") == 19  # 8 ranges of 2^19 vertices
+        d, _ = g.solve(0, "plain")
+        assert np.array_equal(bits(d), bits(oracle.dijkstra(src, dst, w, V, 0)))
+    src, dst, w = S.rmat(16, 16, seed=1, threads=8)
+    with S.DeviceGraph(src, dst, w, 1 << 16) as g:
+        assert g.get_param("blk_range_bits") == 0
 
 
 @pytest.mark.gpu
