@@ -115,6 +115,34 @@ __device__ __forceinline__ void flush_acc(RoundAcc &acc, RoundSlot *s)
     flush_acc_n<kBlock / 64>(acc, s);
 }
 
+// End-of-kernel flush of every wave's staged appends with one queue
+// reservation per workgroup: a returning atomic on the one queue counter per
+// wave, from every wave of a large grid at about the same time, queued at the
+// counter's memory channel for tens of microseconds. Every thread of the
+// workgroup calls it.
+template <int NW>
+__device__ __forceinline__ void stage_flush_block(int32_t (*stage)[kStageCap], int wcnt, int32_t *q,
+                                                  int32_t *len)
+{
+    __shared__ int s_off[NW];
+    __shared__ int s_base;
+    const int w = threadIdx.x >> 6, lane = __lane_id();
+    if (lane == 0) s_off[w] = wcnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int i = 0; i < NW; i++) {
+            const int c = s_off[i];
+            s_off[i] = tot;
+            tot += c;
+        }
+        s_base = tot > 0 ? atomicAdd(len, tot) : 0;
+    }
+    __syncthreads();
+    const int base = s_base + s_off[w];
+    for (int i = lane; i < wcnt; i += 64) q[base + i] = stage[w][i];
+}
+
 // A vertex lowered in this round joins the next frontier when it has
 // out-edges and no other lane appended it in this round (stamp). A plain read
 // of the stamp filters sinks (kSinkStamp) and vertices already appended, so
@@ -144,15 +172,25 @@ __device__ __forceinline__ void append_frontier_n(const bool (&low)[N], const in
             r1[k] = row_ptr[v[k] + 1];
         }
     }
+    // the exchanges are issued together and their results used after all
+    // of them (a use between two made the compiler wait for each in turn)
     bool app[N];
+    int32_t old[N];
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        app[k] = false;
-        if (low[k]) {
-            if constexpr (DIR == DIR_BFS) app[k] = r1[k] > r0[k];
-            else if (sv[k] != tick_cur && sv[k] != kSinkStamp)
-                app[k] = atomicExch(&stamp[v[k]], tick_cur) != tick_cur;
-        }
+        old[k] = tick_cur;
+        if constexpr (DIR != DIR_BFS)
+            if (low[k] && sv[k] != tick_cur && sv[k] != kSinkStamp)
+                old[k] = atomicExch(&stamp[v[k]], tick_cur);
+    }
+    // pins each result in a register after the last exchange, so that the
+    // compares below cannot be moved between the exchanges
+#pragma unroll
+    for (int k = 0; k < N; k++) asm volatile("" : "+v"(old[k]));
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if constexpr (DIR == DIR_BFS) app[k] = low[k] && r1[k] > r0[k];
+        else app[k] = low[k] && old[k] != tick_cur;
     }
 #pragma unroll
     for (int k = 0; k < N; k++) {
@@ -207,7 +245,9 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
     constexpr int TE = kBlock * K;
     constexpr uint32_t INF = WT::INF;
     __shared__ int32_t s_off[TE + 1];
-    __shared__ int32_t s_v[TE + 1];
+    // entry vertex: pull and BFS rounds need it after the edge loads (push
+    // rounds take the edge's own target)
+    __shared__ int32_t s_v[DIR == DIR_PUSH ? 1 : TE + 1];
     __shared__ uint32_t s_d[TE + 1];
     __shared__ int64_t s_start[SRC == SRC_SPARSE ? TE + 1 : 1];
     __shared__ int32_t s_stage[kBlock / 64][kStageCap];
@@ -243,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
             const int32_t j = j0 + jj;
             s_off[jj] = (int32_t)(a.ent_scan[j] - e0);
             const int32_t v = a.ent_v[j];
-            s_v[jj] = v;
+            if constexpr (DIR != DIR_PUSH) s_v[jj] = v;
             if constexpr (SRC == SRC_SPARSE) s_start[jj] = a.ent_start[j];
             const uint32_t dv = a.dist[v];
             bool act;
@@ -302,8 +342,13 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
             const bool act = (DIR == DIR_PULL) ? sk[k] != 0u : sk[k] != INF;
             dk[k] = act ? a.dist[ok[k]] : INF;
         }
+        // Every compare of the K edges is done before the first lowering
+        // atomic is issued (they go out together below): an atomic issued
+        // between two compares made the compiler wait for it -- a trip to the
+        // memory side -- before the next compare, K times per tile.
         bool lowk[K];
         int32_t vk[K];
+        uint32_t ck[K];  // the candidate to send when lowk (push, pull)
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int32_t rel = k * kBlock + tid;
@@ -311,6 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
             const int jj = jk[k];
             bool low = false;
             int32_t v = 0;
+            ck[k] = 0u;
             if constexpr (DIR == DIR_PULL) {
                 const uint32_t dv = sk[k];
                 uint32_t cand = INF;
@@ -330,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                 const bool head = (lane == 0) || (pseg != seg);
                 if (head && dv != 0u && cand < dv) {
                     v = s_v[jj];
-                    lower_dist(a.dist, v, cand);
+                    ck[k] = cand;
                     low = true;
                     acc.minw = min(acc.minw, cand);
                     acc.lowered++;
@@ -347,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
                         if (dv != INF && cand == dv && atomicCAS(&a.parent[v], -1, s_v[jj]) == -1)
                             low = true;
                     } else if (cand < dv && filt_send<kFiltBitsRound>(s_filt, v, cand)) {
-                        lower_dist(a.dist, v, cand);
+                        ck[k] = cand;
                         low = true;
                         acc.minw = min(acc.minw, cand);
                         acc.lowered++;
@@ -357,11 +403,16 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
             lowk[k] = low;
             vk[k] = v;
         }
+        if constexpr (DIR != DIR_BFS) {
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (lowk[k]) lower_dist(a.dist, vk[k], ck[k]);
+        }
         append_frontier_n<DIR, K>(lowk, vk, a.row_ptr, a.stamp, a.tick_cur, acc.qedges, wbuf,
                                   wcnt, a.qnext, &a.slot->qlen);
         __syncthreads();
     }
-    stage_flush(wbuf, wcnt, a.qnext, &a.slot->qlen);
+    stage_flush_block<kBlock / 64>(s_stage, wcnt, a.qnext, &a.slot->qlen);
     flush_acc(acc, a.slot);
 }
 
