@@ -494,26 +494,32 @@ __global__ __launch_bounds__(kBlock) void k_el_active(const int32_t *__restrict_
         }
         bool low[4];
         int32_t vv[4];
+        uint32_t ck[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             low[i] = false;
             vv[i] = 0;
+            ck[i] = 0u;
             if (du[i] != INF) {
                 vv[i] = et[i];
                 const uint32_t cand = WT::add(du[i], ew[i]);
                 acc.relaxed++;
                 if (cand < dv[i] && filt_send<kFiltBitsRound>(s_filt, vv[i], cand)) {
-                    lower_dist(dist, vv[i], cand);
+                    ck[i] = cand;
                     low[i] = true;
                     acc.minw = min(acc.minw, cand);
                     acc.lowered++;
                 }
             }
         }
+        // the lowering atomics after every compare (see k_advance)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (low[i]) lower_dist(dist, vv[i], ck[i]);
         append_frontier_n<DIR_PUSH, 4>(low, vv, row_ptr, stamp, tick_cur, acc.qedges, wbuf, wcnt,
                                        qnext, &slot->qlen);
     }
-    stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+    stage_flush_block<kBlock / 64>(s_stage, wcnt, qnext, &slot->qlen);
     flush_acc(acc, slot);
 }
 
@@ -1147,7 +1153,7 @@ __global__ __launch_bounds__(kBlock) void k_lowered_compact(const uint32_t *dist
         stage_push(app, (int32_t)v, wbuf, wcnt);
         if (wcnt > kStageCap - 64) stage_flush(wbuf, wcnt, qnext, &slot->qlen);
     }
-    stage_flush(wbuf, wcnt, qnext, &slot->qlen);
+    stage_flush_block<kBlock / 64>(s_stage, wcnt, qnext, &slot->qlen);
     flush_acc(acc, slot);
 }
 
