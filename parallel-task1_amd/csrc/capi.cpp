@@ -40,7 +40,7 @@ void graph_free(sssp_graph *g)
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     void *ptrs[] = {g->row_ptr, g->el_src, g->el_dst, g->el_w, g->col_ptr, g->csc_src, g->csc_w,
-                    g->nzo_v, g->nzo_scan, g->nzi_v, g->nzi_scan, g->sample_v, g->dist, g->stamp,
+                    g->nzo_v, g->nzo_scan, g->nzi_v, g->nzi_scan, g->sample_v, g->sample_in, g->dist, g->stamp,
                     g->queue[0], g->queue[1], g->lb_v, g->lb_start, g->lb_scan, g->lb_tile,
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
@@ -205,6 +205,7 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
         sv[(size_t)i] = (int32_t)(((int64_t)i * g->V) / g->n_sample);
     g->sample_v = dmalloc<int32_t>((size_t)g->n_sample);
     SSSP_HIP(hipMemcpyAsync(g->sample_v, sv.data(), 4 * sv.size(), hipMemcpyHostToDevice, s));
+    sample_in_degrees(g);
     SSSP_HIP(hipEventCreate(&g->ev_a));
     SSSP_HIP(hipEventCreate(&g->ev_b));
     SSSP_HIP(hipEventCreate(&g->ev_r0));

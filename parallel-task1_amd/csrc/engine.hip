@@ -834,10 +834,11 @@ __global__ __launch_bounds__(kBlock) void k_init_sources(Ctrl *c, uint32_t *dist
 // (distance above the frontier minimum) and unreached fractions of a fixed
 // vertex sample, with the in-degree of the unsettled ones (pull estimate).
 __device__ __forceinline__ void round_sample(Ctrl *c, int32_t r, int pile,
-                                             const int32_t *sample_v, int32_t n_sample,
-                                             const uint32_t *dist, const int64_t *col_ptr,
-                                             uint32_t inf)
+                                             const int32_t *sample_v, const int32_t *sample_in,
+                                             int32_t n_sample, const uint32_t *dist, uint32_t inf)
 {
+    __shared__ int s_cnt[kBlock / 64], s_unr[kBlock / 64];
+    __shared__ int64_t s_ind[kBlock / 64];
     RoundSlot *cur = &c->slot[r & 1];
     RoundSlot *nxt = &c->slot[(r + 1) & 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -850,17 +851,33 @@ __device__ __forceinline__ void round_sample(Ctrl *c, int32_t r, int pile,
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n_sample;
          i += (int64_t)gridDim.x * kBlock) {
         const int32_t v = sample_v[i];
+        const int32_t din = sample_in ? sample_in[i] : 0;  // loaded beside v
         const uint32_t d = dist[v];
         if (d > minF) {
             cnt++;
-            if (col_ptr) ind += col_ptr[v + 1] - col_ptr[v];
+            ind += din;
         }
         unr += d == inf ? 1 : 0;
     }
+    // one set of counter atomics per workgroup
     cnt = wave_sum32(cnt);
     unr = wave_sum32(unr);
     ind = wave_sum64(ind);
+    const int w = threadIdx.x >> 6;
     if (__lane_id() == 0) {
+        s_cnt[w] = cnt;
+        s_unr[w] = unr;
+        s_ind[w] = ind;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cnt = unr = 0;
+        ind = 0;
+        for (int i = 0; i < kBlock / 64; i++) {
+            cnt += s_cnt[i];
+            unr += s_unr[i];
+            ind += s_ind[i];
+        }
         if (cnt) atomicAdd(&cur->sample_unsettled, cnt);
         if (unr) atomicAdd(&cur->sample_unreached, unr);
         if (ind) atomicAdd((unsigned long long *)&cur->sample_unsettled_in, (unsigned long long)ind);
@@ -868,11 +885,22 @@ __device__ __forceinline__ void round_sample(Ctrl *c, int32_t r, int pile,
 }
 
 __global__ __launch_bounds__(kBlock) void k_round_begin(Ctrl *c, int32_t r, int pile,
-                                                        const int32_t *sample_v, int32_t n_sample,
-                                                        const uint32_t *dist,
-                                                        const int64_t *col_ptr, uint32_t inf)
+                                                        const int32_t *sample_v,
+                                                        const int32_t *sample_in, int32_t n_sample,
+                                                        const uint32_t *dist, uint32_t inf)
 {
-    round_sample(c, r, pile, sample_v, n_sample, dist, col_ptr, inf);
+    round_sample(c, r, pile, sample_v, sample_in, n_sample, dist, inf);
+}
+
+// In-degree of each sample vertex (the pull estimate's weight), once per graph.
+__global__ __launch_bounds__(kBlock) void k_sample_in(const int32_t *sample_v, int32_t n,
+                                                      const int64_t *col_ptr, int32_t *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = sample_v[i];
+    const int64_t d = col_ptr[v + 1] - col_ptr[v];
+    out[i] = (int32_t)min(d, (int64_t)0x7FFFFFFF);
 }
 
 struct DecideArgs {
@@ -1004,12 +1032,13 @@ __device__ __forceinline__ void decide_round(Ctrl *c, const DecideArgs &d)
 // sees every other workgroup's sample counts through the fences around the
 // counter -- resets the counter and decides.
 __global__ __launch_bounds__(kBlock) void k_round_open(Ctrl *c, int32_t r, int pile,
-                                                       const int32_t *sample_v, int32_t n_sample,
-                                                       const uint32_t *dist, const int64_t *col_ptr,
-                                                       uint32_t inf, DecideArgs d)
+                                                       const int32_t *sample_v,
+                                                       const int32_t *sample_in, int32_t n_sample,
+                                                       const uint32_t *dist, uint32_t inf,
+                                                       DecideArgs d)
 {
     __shared__ int s_last;
-    round_sample(c, r, pile, sample_v, n_sample, dist, col_ptr, inf);
+    round_sample(c, r, pile, sample_v, sample_in, n_sample, dist, inf);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) s_last = atomicAdd(&c->open_done, 1) == (int)gridDim.x - 1;
@@ -1834,9 +1863,9 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     da.pile = g->pile_cur;
     da.mirror = g->ctrl_host_dev;
     da.seq = ++g->dec_seq;
-    k_round_open<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v, ns,
-                                                     g->dist, g->has_csc ? g->col_ptr : nullptr,
-                                                     WT::INF, da);
+    k_round_open<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v,
+                                                     g->has_csc ? g->sample_in : nullptr, ns,
+                                                     g->dist, WT::INF, da);
     SSSP_LAUNCH_CHECK("k_round_open");
     wait_decision(g, da.seq);
     const RoundSlot cur = g->ctrl_host->slot[r & 1];
@@ -2016,7 +2045,7 @@ void certify_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int64_t *co
     SSSP_LAUNCH_CHECK("k_cert_roots");
     for (int32_t r = 0;; r++) {
         if (r > g->V + 2) throw Error{SSSP_ERR_INTERNAL, "certificate BFS did not terminate"};
-        k_round_begin<<<1, kBlock, 0, s>>>(c, r, -1, g->sample_v, 0, g->dist, nullptr,
+        k_round_begin<<<1, kBlock, 0, s>>>(c, r, -1, g->sample_v, nullptr, 0, g->dist,
                                            0xFFFFFFFFu);
         SSSP_LAUNCH_CHECK("k_round_begin (bfs)");
         read_slot(g, r);
@@ -2232,6 +2261,17 @@ void reset_stamps(sssp_graph *g)
     k_fill<int32_t><<<grid_for(g->V), kBlock, 0, g->stream>>>(g->farst, g->V, 0);
     SSSP_LAUNCH_CHECK("pile state reset");
     g->pile_dirty = false;
+}
+
+void sample_in_degrees(sssp_graph *g)
+{
+    dfree(g->sample_in);
+    g->sample_in = nullptr;
+    if (!g->has_csc || g->n_sample <= 0) return;
+    g->sample_in = dmalloc<int32_t>((size_t)g->n_sample);
+    k_sample_in<<<(unsigned)((g->n_sample + kBlock - 1) / kBlock), kBlock, 0, g->stream>>>(
+        g->sample_v, g->n_sample, g->col_ptr, g->sample_in);
+    SSSP_LAUNCH_CHECK("k_sample_in");
 }
 
 void unpermute_rows(const uint32_t *in, const int32_t *vperm, int64_t V, int64_t rows,

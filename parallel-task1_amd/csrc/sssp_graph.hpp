@@ -198,6 +198,7 @@ struct sssp_graph {
     int32_t *nzi_tile[sssp::kChunkCount] = {};
     // vertices sampled for the unsettled-fraction estimate
     int32_t *sample_v = nullptr;
+    int32_t *sample_in = nullptr;  // in-degree of each sample vertex (with the CSC)
     int32_t n_sample = 0;
 
     // vertex order: the layouts and every per-vertex array index vertices by
@@ -377,6 +378,8 @@ void build_tile_table(const int64_t *scan, int32_t n, int64_t total, int32_t til
                       int32_t *tile_out, hipStream_t s);
 
 // engine.hip
+// in-degrees of the sample vertices (after the layouts and the sample)
+void sample_in_degrees(sssp_graph *g);
 // out[r * V + v] = in[r * V + vperm[v]] for rows r < rows: caller order from
 // slot order (in and out must not overlap)
 void unpermute_rows(const uint32_t *in, const int32_t *vperm, int64_t V, int64_t rows,
