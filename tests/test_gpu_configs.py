@@ -93,6 +93,7 @@ def test_c2_rmat22_ef32_sixteen_sources(gpu, oracle):
             for sched in t:
                 g.set_param("plain_kernel", 1 if sched == "plain_edge_list" else 0)
                 d, st = g.solve(int(s), "plain" if sched == "plain_edge_list" else sched)
+                g.set_param("plain_kernel", 0)
                 t[sched].append(st["t_solve_ms"])
                 if ref is None:
                     ref = d
@@ -255,6 +256,7 @@ def test_c5_rmat25_learned_heuristic_plain(gpu, oracle):
             for sched in t:
                 g.set_param("plain_kernel", 1 if sched == "plain_edge_list" else 0)
                 d, st = g.solve(int(s), "plain" if sched == "plain_edge_list" else sched)
+                g.set_param("plain_kernel", 0)
                 t[sched].append(st["t_solve_ms"])
                 if ref is None:
                     ref = d
