@@ -638,12 +638,6 @@ void build_static_list(const int64_t *ptr, int32_t V, int64_t E, int64_t *tmp_fl
 
 }  // namespace
 
-void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s)
-{
-    k_scan_blocks<<<1, kBlock, 0, s>>>(blk, nb);
-    SSSP_LAUNCH_CHECK("k_scan_blocks");
-}
-
 void scan_blocks_pair_inplace(int64_t *a, int64_t *b, int64_t nb, hipStream_t s)
 {
     k_scan_blocks_pair<<<2, kBlock, 0, s>>>(a, b, nb);

@@ -27,7 +27,7 @@ struct alignas(64) RoundSlot {
     int64_t qedges;        // sum of out-degrees of appended vertices
     int64_t relaxed;       // edge relaxations evaluated
     int32_t lowered;       // relaxations whose candidate beat the distance read
-    // filled by the round-begin kernel for the consuming round:
+    // filled by the round's sample (k_round_open) for the consuming round:
     int32_t sample_unreached;   // sampled vertices with d = INF
     int32_t sample_unsettled;   // sampled vertices with d > minF
     int32_t sample_n;           // sample size
@@ -357,8 +357,8 @@ void rmat_device(int scale, int ef, int a, int b, int c, int d, uint32_t base_se
 // counters to use, or null (allocated for the call)
 void validate_edges(const int32_t *src, const int32_t *dst, const void *w, int wtype, int64_t E,
                     int32_t V, hipStream_t s, int64_t *bad_buf = nullptr);
-void scan_blocks_inplace(int64_t *blk, int64_t nb, hipStream_t s);
-// the same for two arrays in one launch
+// in-place exclusive scan of two arrays of nb + 1 values (the last one
+// receives the total), one workgroup each, in one launch
 void scan_blocks_pair_inplace(int64_t *a, int64_t *b, int64_t nb, hipStream_t s);
 // packed 12-byte binary edge-list records (device) -> src, dst, w (device)
 void split_records(const uint32_t *rec, int64_t n, int32_t *src, int32_t *dst, uint32_t *w,
