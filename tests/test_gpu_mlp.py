@@ -136,6 +136,41 @@ def test_trained_network_against_heuristic(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_trained_bfs_network_against_heuristic(gpu):
+    """The same for the BFS network, on R-MAT scale 24 seed 3 (training used
+    seed 7 and uniform graphs): 4 sources, interleaved, best of three; the
+    levels are identical and the learned BFS is not slower than the rule."""
+    bfs_weights = os.path.join(os.path.dirname(WEIGHTS), "scheduler_bfs_mlp.f32")
+    if not os.path.exists(bfs_weights):
+        pytest.skip("no trained BFS weights")
+    g = S.DeviceGraph.rmat(24, 16, seed=3, threads=64)
+    try:
+        srcs = S.pick_sources(np.nonzero(g.out_degrees())[0].astype(np.int32), g.V, 3, 4)
+        m = S.MLP(0)
+        m.set_params(np.fromfile(bfs_weights, np.float32))
+        g.set_bfs_mlp(m)
+        th, tl = [], []
+        for s in srcs:
+            _, lh, _ = g.bfs(int(s), "heuristic")
+            _, ll, sl = g.bfs(int(s), "learned")
+            assert np.array_equal(lh, ll)
+            assert sl["mlp_rounds"] > 0
+            bh = bl = float("inf")
+            for _ in range(3):
+                bh = min(bh, g.bfs(int(s), "heuristic", levels=False)[2]["t_solve_ms"])
+                bl = min(bl, g.bfs(int(s), "learned", levels=False)[2]["t_solve_ms"])
+            th.append(bh)
+            tl.append(bl)
+        gain = sum(th) / sum(tl)
+        print(f"learned vs heuristic BFS on held-out RMAT-24: {gain:.3f}x (heuristic {sum(th):.2f} ms, "
+              f"learned {sum(tl):.2f} ms over {len(srcs)} sources)")
+        assert gain >= 1.0, f"learned BFS {gain:.3f}x the rule"
+    finally:
+        g.close()
+
+
+@pytest.mark.gpu
 def test_bfs_network_slot(gpu, oracle):
     """BFS has its own network slot: without one a learned BFS runs the rule
     (the SSSP network is not used); with any network the levels stay the
