@@ -61,6 +61,7 @@ extern "C" {
 #define SSSP_MODE_PULL  2
 #define SSSP_MODE_CHECK 3  /* trace only: the final fixpoint check (run dense) */
 #define SSSP_MODE_DENSE 4  /* trace only: an all-edge round run dense (see dense_all_frac) */
+#define SSSP_MODE_TAIL  5  /* trace only: a push round run as a tail round (see tail_edges) */
 
 /* ---- graph creation flags ------------------------------------------------ */
 #define SSSP_INPUT_DEVICE 1u   /* src/dst/w are device pointers on `device` */
@@ -184,7 +185,8 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "delta", "hub_degree", "plain_below_edges", "rollout_policy", "collect_kind",
  * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
  * "batch_capture", "decide_spin", "blk_tile_cap", "blk_tile_order",
- * "blk_range_bits", "plain_kernel", "trace_events"; "cu_count",
+ * "blk_range_bits", "plain_kernel", "tail_edges", "tail_group", "tail_blocks",
+ * "trace_events"; "cu_count",
  * "collect_rounds", "multi_passes", "blk_tiles", "blk_wide_chunks" and
  * "plain_blocked" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
@@ -211,6 +213,15 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * tiles or more, else the XCD-partitioned edge-list pass; 1 always the
  * edge-list pass, 2 always the blocked pass. "plain_blocked" (read-only) = 1
  * when plain solves use the blocked pass.
+ * tail_edges (default 2^23): once a heuristic or learned solve's frontier has
+ * had more than tail_edges out-edges, a round decided as a push round over a
+ * frontier with at most that many (far pile off) starts the tail rounds: push
+ * rounds queued tail_group (default 4) at a time with no decision or host
+ * round trip between them (three launches each: entry list, relaxation over
+ * tail_blocks workgroups -- 0 = 2 per CU --, bookkeeping), until the frontier
+ * is empty or has more out-edges than tail_edges again; 0 turns them off.
+ * Traces show them as SSSP_MODE_TAIL, with sizes and times only under
+ * trace_events (which queues them one at a time).
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with
  * d >= (min distance over the frontier and pile) + delta and out-degree >=
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.

@@ -618,6 +618,15 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "hub_degree") {
             SSSP_REQUIRE(value >= 0 && value <= 2147483647.0, "hub_degree must be in [0, 2^31)");
             g->hub_degree = (int)value;
+        } else if (n == "tail_edges") {
+            SSSP_REQUIRE(value >= 0 && value <= 9.2e18, "tail_edges must be >= 0");
+            g->tail_edges = (int64_t)value;
+        } else if (n == "tail_group") {
+            SSSP_REQUIRE(value >= 1 && value <= 64, "tail_group must be in [1, 64]");
+            g->tail_group = (int)value;
+        } else if (n == "tail_blocks") {
+            SSSP_REQUIRE(value >= 0 && value <= 65536, "tail_blocks must be in [0, 65536]");
+            g->tail_blocks = (int)value;
         } else if (n == "trace_events") g->trace_events = value != 0;
         else throw Error{SSSP_ERR_ARG, "unknown parameter '" + n + "'"};
     });
@@ -651,6 +660,9 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "blk_range_bits") *value = g->blk_range_bits;
         else if (n == "plain_kernel") *value = g->plain_kernel;
         else if (n == "plain_blocked") *value = plain_uses_blocked(g) ? 1 : 0;
+        else if (n == "tail_edges") *value = (double)g->tail_edges;
+        else if (n == "tail_group") *value = g->tail_group;
+        else if (n == "tail_blocks") *value = g->tail_blocks;
         else if (n == "multi_passes") *value = g->multi_passes;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;

@@ -65,6 +65,8 @@ struct AdvArgs {
     uint32_t minF;
     const int64_t *dev_n;      // when set, entry count and edge total are read
     const int64_t *dev_T;      // from the device (near-far compaction)
+    const unsigned long long *dev_nt;  // when set, both from one word: count << 32 | total
+                                       // (tail rounds)
 };
 
 constexpr int kStageCap = 512;  // per-wave LDS staging slots for frontier appends
@@ -262,6 +264,11 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
         a.n = (int32_t)*a.dev_n;
         a.T = *a.dev_T;
         a.ntiles = (a.T + TE - 1) / TE;
+    } else if (a.dev_nt) {
+        const unsigned long long nt = *a.dev_nt;
+        a.n = (int32_t)(nt >> 32);
+        a.T = (int64_t)(nt & 0xFFFFFFFFull);
+        a.ntiles = (a.T + TE - 1) / TE;
     }
 
     // XCD-aware tile order: groups of 16 consecutive tiles stay on one XCD
@@ -276,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
         const int64_t e0 = t * TE;
         const int64_t e1 = min(e0 + (int64_t)TE, a.T);
         const int32_t j0 = a.tile[t];
-        const int32_t jl = min(a.tile[t + 1], a.n - 1);
+        const int32_t jl = t + 1 < a.ntiles ? min(a.tile[t + 1], a.n - 1) : a.n - 1;
         const int m = jl - j0 + 1;
         int any = 0;
         for (int jj = tid; jj < m; jj += kBlock) {
@@ -798,6 +805,10 @@ __global__ void k_init_solve(Ctrl *c, uint32_t *dist, int32_t *stamp, int32_t *q
         c->pile_edges[i] = 0;
     }
     c->lb_n = 0;
+    c->tail_stop = -1;
+    c->tail_rounds = 0;
+    c->tail_relaxed = 0;
+    c->tail_list = 0;
     if (source < 0) return;
     dist[source] = 0u;
     const int64_t deg = row_ptr[source + 1] - row_ptr[source];
@@ -1047,6 +1058,135 @@ __global__ __launch_bounds__(kBlock) void k_round_open(Ctrl *c, int32_t r, int p
     __threadfence();
     if (threadIdx.x == 0) c->open_done = 0;
     decide_round(c, d);
+}
+
+// ---------------------------------------------------------------------------
+// Tail rounds: push rounds over a small frontier, queued several at a time
+// with no decision between them. Once the frontier has shrunk to few
+// out-edges, the rounds left are short (tens of microseconds of work), and
+// each decided round would also pay its decision launch, the host's round
+// trip and three load-balancing launches. A tail round is three launches that
+// read their frontier from the device: k_tail_list builds the load-balancing
+// entries in one pass, k_advance (the push rounds' kernel) relaxes, and
+// k_tail_close (one thread) does the bookkeeping. A tail round runs only while
+// the rounds before it ran and its frontier is non-empty with at most
+// tail_edges out-edges; the first one that does not is where the host's
+// decided rounds resume (tail_stop).
+// ---------------------------------------------------------------------------
+constexpr int kTailChunk = 2;  // chunk bucket of the tail rounds' relaxation (4 edges per lane)
+
+struct TailArgs {
+    Ctrl *ctrl;
+    Ctrl *mirror;          // device address of the host mirror (coherent host memory)
+    const int64_t *row_ptr;
+    const int32_t *adj;    // CSR targets (src-sorted edge list)
+    const uint32_t *wts;
+    uint32_t *dist;
+    int32_t *stamp;
+    const int32_t *q;      // frontier this round consumes
+    int32_t *qnext;        // frontier it builds
+    int64_t tail_edges;    // out-edge bound of a frontier a tail round takes
+    int32_t r;             // round index: consumes slot r & 1, fills slot (r + 1) & 1
+    int32_t tick_cur;
+    int32_t seq;           // written to mirror->dec_seq once the mirror holds the round
+    int32_t first;         // 1: the first round of a run of tail rounds
+};
+
+// Whether tail round a.r runs; every workgroup of its kernels reads the same
+// values (written by earlier kernels).
+__device__ __forceinline__ bool tail_runs(const TailArgs &a)
+{
+    const Ctrl *c = a.ctrl;
+    const RoundSlot *in = &c->slot[a.r & 1];
+    return (a.first || c->tail_stop < 0) && in->qlen > 0 && in->qedges <= a.tail_edges;
+}
+
+// The entry list of a tail round, built without a separate scan: each wave
+// takes 64 frontier vertices, scans their out-degrees across its lanes and
+// reserves its entries and their edges with one 64-bit atomic (count << 32 |
+// edges), so entries stay in edge order across waves. Writes the entries
+// (vertex, row start, exclusive edge offset) and, for each tile of TE edges,
+// the entry holding its first edge -- the load-balancing lists k_advance
+// reads (k_lb_reduce / k_lb_write build the same for decided rounds).
+template <class WT>
+__global__ __launch_bounds__(kBlock) void k_tail_list(TailArgs a, int32_t *ent_v,
+                                                      int64_t *ent_start, int64_t *ent_scan,
+                                                      int32_t *tile, int64_t TE)
+{
+    if (!tail_runs(a)) return;  // grid-uniform
+    const int32_t n = a.ctrl->slot[a.r & 1].qlen;
+    const int lane = __lane_id();
+    const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+    for (int64_t b = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 64; b < n;
+         b += nw * 64) {
+        const int64_t j = b + lane;
+        int32_t v = 0;
+        int64_t r0 = 0, deg = 0;
+        if (j < n) {
+            v = a.q[j];
+            const uint32_t d = a.dist[v];
+            const int64_t p0 = a.row_ptr[v], p1 = a.row_ptr[v + 1];
+            if (d != WT::INF) {
+                r0 = p0;
+                deg = p1 - p0;
+            }
+        }
+        const bool has = deg > 0;
+        const uint64_t m = __ballot(has);
+        if (m == 0) continue;  // wave-uniform
+        int64_t inc = deg;  // inclusive scan of the degrees over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        const int64_t tot = __shfl(inc, 63);
+        unsigned long long old = 0;
+        if (lane == 0)
+            old = atomicAdd(&a.ctrl->tail_list,
+                            ((unsigned long long)__popcll(m) << 32) | (unsigned long long)tot);
+        old = __shfl(old, 0);
+        if (has) {
+            const int64_t slot = (int64_t)(old >> 32) + __popcll(m & lanemask_lt());
+            const int64_t off = (int64_t)(old & 0xFFFFFFFFull) + inc - deg;
+            ent_v[slot] = v;
+            ent_start[slot] = r0;
+            ent_scan[slot] = off;
+            for (int64_t t = (off + TE - 1) / TE; t * TE < off + deg; t++) tile[t] = (int32_t)slot;
+        }
+    }
+}
+
+// Bookkeeping of tail round a.r (one thread): a round that ran resets its
+// input slot (the one round a.r + 1 fills) and adds to the run's totals; the
+// first round that did not run records itself as tail_stop. Reports to the
+// host mirror, the sequence number last.
+__global__ void k_tail_close(TailArgs a)
+{
+    if (threadIdx.x != 0) return;
+    Ctrl *c = a.ctrl;
+    Ctrl *m = a.mirror;
+    const bool ran = tail_runs(a);
+    if (a.first) {
+        c->tail_stop = -1;
+        c->tail_rounds = 0;
+        c->tail_relaxed = 0;
+    }
+    if (ran) {
+        const RoundSlot o = c->slot[(a.r + 1) & 1];
+        zero_slot(&c->slot[a.r & 1], 0xFFFFFFFFu);
+        c->tail_list = 0;
+        c->tail_rounds += 1;
+        c->tail_relaxed += o.relaxed;
+        m->slot[(a.r + 1) & 1] = o;
+    } else if (c->tail_stop < 0) {
+        c->tail_stop = a.r;
+    }
+    m->tail_stop = c->tail_stop;
+    m->tail_rounds = c->tail_rounds;
+    m->tail_relaxed = c->tail_relaxed;
+    __threadfence_system();
+    __hip_atomic_store(&m->dec_seq, a.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---------------------------------------------------------------------------
@@ -1836,13 +1976,112 @@ void wait_decision(sssp_graph *g, int32_t seq)
     SSSP_HIP(hipStreamSynchronize(g->stream));
 }
 
-// One round r: bookkeeping and decision (k_round_open, one launch), host
-// read-back of the frontier counters, then the round itself. Returns false, having run nothing, when
-// the solve is over (no frontier, no pile, fixpoint check done or off).
-// force_mode >= 0 replaces the decision (not the fixpoint check).
+// Tail rounds from round r on, tail_group queued per host check, until one
+// does not run (empty frontier, or more than tail_edges out-edges again) or
+// the round cap. Returns the index of the first round not run here; its input
+// slot is in the host mirror.
 template <class WT>
-bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *st,
-                bool &verified, int64_t &relaxed, int force_mode = -1, int force_chunk = 0)
+int32_t run_tail(sssp_graph *g, int32_t r, int32_t tick0, int32_t r_cap, sssp_stats *st,
+                 int64_t &relaxed)
+{
+    hipStream_t s = g->stream;
+    const int group = g->trace_events ? 1 : std::max(1, g->tail_group);
+    // the list kernel's grid covers 2^17 frontier vertices per pass; the
+    // relaxation's, tail_blocks workgroups (0 = 2 per CU) looping over tiles
+    // (measured at R-MAT 25 x 16 and 22 x 32: 2 per CU 1-2 % faster than 4)
+    const unsigned grid = (unsigned)std::max(1, g->cu_count * 2);
+    const int64_t nb = g->tail_blocks > 0 ? g->tail_blocks : (int64_t)g->cu_count * 2;
+    const unsigned agrid = (unsigned)std::max<int64_t>(8, (nb + 7) / 8 * 8);
+    const int64_t TE = (int64_t)kBlock * kChunkK[kTailChunk];
+    TailArgs a{};
+    a.ctrl = g->ctrl;
+    a.mirror = g->ctrl_host_dev;
+    a.row_ptr = g->row_ptr;
+    a.adj = g->el_dst;
+    a.wts = g->el_w;
+    a.dist = g->dist;
+    a.stamp = g->stamp;
+    a.tail_edges = g->tail_edges;
+    a.first = 1;
+    const Ctrl &h = *g->ctrl_host;
+    const int32_t r0 = r;
+    for (;;) {
+        const int n = (int)std::min<int64_t>(group, (int64_t)r_cap - r);
+        if (n <= 0) break;
+        const RoundSlot in = h.slot[r & 1];  // the frontier the first queued round consumes
+        if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
+        for (int i = 0; i < n; i++) {
+            a.r = r + i;
+            a.q = g->queue[a.r & 1];
+            a.qnext = g->queue[(a.r + 1) & 1];
+            a.tick_cur = tick0 + a.r + 1;
+            a.seq = ++g->dec_seq;
+            k_tail_list<WT><<<grid, kBlock, 0, s>>>(a, g->lb_v, g->lb_start, g->lb_scan,
+                                                     g->lb_tile, TE);
+            SSSP_LAUNCH_CHECK("k_tail_list");
+            AdvArgs adv = base_args<WT>(g, a.r, tick0);
+            adv.ent_v = g->lb_v;
+            adv.ent_start = g->lb_start;
+            adv.ent_scan = g->lb_scan;
+            adv.tile = g->lb_tile;
+            adv.adj = g->el_dst;
+            adv.wts = g->el_w;
+            adv.dev_nt = &g->ctrl->tail_list;
+            adv.n = 1;  // the counts are read on the device
+            adv.T = 1;
+            adv.ntiles = 1;
+            launch_adv_k<WT, DIR_PUSH, SRC_SPARSE, PRED_NONE>(kTailChunk, agrid, s, adv);
+            k_tail_close<<<1, 64, 0, s>>>(a);
+            SSSP_LAUNCH_CHECK("k_tail_close");
+            a.first = 0;
+        }
+        if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r1, s));
+        wait_decision(g, a.seq);
+        if (h.tail_stop >= 0) {
+            r = h.tail_stop;
+            break;
+        }
+        if (g->trace_events) {
+            float ms = 0.f;
+            SSSP_HIP(hipEventSynchronize(g->ev_r1));
+            SSSP_HIP(hipEventElapsedTime(&ms, g->ev_r0, g->ev_r1));
+            TraceRow tr{};
+            tr.mode = SSSP_MODE_TAIL;
+            tr.frontier = in.qlen;
+            tr.frontier_edges = in.qedges;
+            tr.ms = ms;
+            g->trace.push_back(tr);
+        }
+        r += n;
+    }
+    // totals of the rounds run here; the last one's relaxations are in the
+    // slot the next decided round reads, which counts them
+    const int32_t done = r > r0 ? h.tail_rounds : 0;
+    if (done > 0) relaxed += h.tail_relaxed - h.slot[r & 1].relaxed;
+    if (!g->trace_events) {
+        // one row per round (sizes and times are only traced with
+        // trace_events: rounds queued together are not told apart)
+        TraceRow tr{};
+        tr.mode = SSSP_MODE_TAIL;
+        tr.frontier = tr.frontier_edges = -1;
+        for (int32_t i = 0; i < done; i++) g->trace.push_back(tr);
+    }
+    st->iterations += done;
+    st->n_push += done;
+    return r;
+}
+
+// One round r: bookkeeping and decision (k_round_open, one launch), host
+// read-back of the frontier counters, then the round itself -- or, for a
+// decided push round over a frontier with few out-edges, the tail rounds
+// (run_tail, below r_cap). Returns the number of round indices used, 0
+// (having run nothing) when the solve is over (no frontier, no pile,
+// fixpoint check done or off). force_mode >= 0 replaces the decision (not the
+// fixpoint check) and runs one round.
+template <class WT>
+int32_t step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *st,
+                   bool &verified, int64_t &relaxed, int32_t r_cap, int force_mode = -1,
+                   int force_chunk = 0)
 {
     hipStream_t s = g->stream;
     const bool learned = (sched == SSSP_SCHED_LEARNED);
@@ -1874,7 +2113,7 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     relaxed += cur.relaxed;
     bool verify = false;
     if (cur.qlen == 0 && piled == 0) {
-        if (!g->verify || verified) return false;
+        if (!g->verify || verified) return 0;
         verify = true;
     }
     int mode, kidx;
@@ -1910,6 +2149,18 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     } else {
         verified = false;
     }
+    // the tail rounds take over once the frontier, having had more than
+    // tail_edges out-edges, is down to at most that (a frontier still growing
+    // from the source may hold hubs, whose edges one wave would walk alone)
+    if (cur.qedges + piled_edges > g->tail_edges) g->tail_armed = true;
+    if (!verify && mode == SSSP_MODE_PUSH && force_mode < 0 && g->tail_edges > 0 &&
+        g->tail_armed && (sched == SSSP_SCHED_HEURISTIC || sched == SSSP_SCHED_LEARNED) &&
+        piled == 0 && g->delta <= 0.0 && cur.qlen > 0 && cur.qedges <= g->tail_edges &&
+        r + 1 < r_cap) {
+        const int32_t r1 = run_tail<WT>(g, r, tick0, r_cap, st, relaxed);
+        if (r1 > r) return r1 - r;
+        // (the first tail round always runs: its frontier was checked here)
+    }
     if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
     if (run_round<WT>(g, mode, kidx, r, tick0, cur, verify)) g->pile_cur ^= 1;
     float ms = 0.f;
@@ -1930,21 +2181,29 @@ bool step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stats *
     if (mode == SSSP_MODE_PUSH) st->n_push++;
     else if (mode == SSSP_MODE_ALL) st->n_all++;
     else st->n_pull++;
-    return true;
+    return 1;
 }
 
+// Returns the number of round indices the solve used (its stamp ticks end
+// at tick0 + that number).
 template <class WT>
-void solve_rounds(sssp_graph *g, int sched, int32_t tick0, sssp_stats *st)
+int32_t solve_rounds(sssp_graph *g, int sched, int32_t tick0, sssp_stats *st)
 {
     bool verified = false;
     int64_t relaxed = 0;
     g->pile_cur = 0;
-    for (int32_t r = 0;; r++) {
+    g->tail_armed = false;
+    int32_t r = 0;
+    for (;;) {
         if (r >= g->max_rounds)
             throw Error{SSSP_ERR_NOT_CONVERGED, "solve did not converge within max_rounds"};
-        if (!step_round<WT>(g, r, sched, tick0, st, verified, relaxed)) break;
+        const int32_t used = step_round<WT>(g, r, sched, tick0, st, verified, relaxed,
+                                            g->max_rounds);
+        if (used == 0) break;
+        r += used;
     }
     st->edges_relaxed = relaxed;
+    return r;
 }
 
 void upload_sources(sssp_graph *g, const int32_t *sources, int32_t n)
@@ -2001,14 +2260,15 @@ void solve_t(sssp_graph *g, const int32_t *sources, int32_t n_src, int sched, ss
             SSSP_LAUNCH_CHECK("pile state reset");
         }
         g->pile_dirty = true;
+        int32_t used = 0;
         try {
-            solve_rounds<WT>(g, sched, tick0, st);
+            used = solve_rounds<WT>(g, sched, tick0, st);
         } catch (...) {
             g->tick = tick0 + g->max_rounds + 2;
             throw;
         }
         g->pile_dirty = false;
-        g->tick = tick0 + st->iterations + 2;
+        g->tick = tick0 + used + 2;
     }
     // the result in the caller's vertex order is part of the solve: timed
     if (g->vperm) unpermute_rows(g->dist, g->vperm, g->V, 1, g->dout, s);
@@ -2141,6 +2401,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
     int64_t relaxed = 0;
     sssp_stats st{};
     g->collect_rounds = 0;
+    g->tail_armed = false;
     auto charge = [&] {
         if (++g->collect_rounds > g->collect_budget) {
             SSSP_HIP(hipStreamSynchronize(s));
@@ -2167,8 +2428,10 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
         copy(sq, g->queue[r & 1], 4 * (V + 1));
         copy(sp, g->pile[p], 4 * V);
         copy(sc, g->ctrl, sizeof(Ctrl));
+        const bool armed = g->tail_armed;
         auto restore = [&] {
             g->snap_round = -1;
+            g->tail_armed = armed;
             copy(g->dist, sd, 4 * V);
             copy(g->stamp, sst, 4 * V);
             copy(g->farst, sfa, 4 * V);
@@ -2194,8 +2457,8 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
                         int64_t rel = 0;
                         sssp_stats tmp{};
                         charge();
-                        if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &tmp, ver, rel, mode,
-                                            c)) {
+                        if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &tmp, ver, rel,
+                                            2 * g->max_rounds, mode, c)) {
                             over = true;  // nothing pending at round r: the solve is complete
                             break;
                         }
@@ -2205,7 +2468,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
                             verify_round = g->trace.back().mode == 3;
                         }
                         int32_t r2 = r + 1;
-                        for (;; r2++) {
+                        for (;;) {
                             if (r2 >= 2 * g->max_rounds) {
                                 g->tick = tick0 + 2 * g->max_rounds + 2;
                                 g->trace_events = trace_was;
@@ -2213,8 +2476,10 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
                                             "collect: rollout did not converge"};
                             }
                             charge();
-                            if (!step_round<WT>(g, r2, cont, tick0, &tmp, ver, rel))
-                                break;
+                            const int32_t used = step_round<WT>(g, r2, cont, tick0, &tmp, ver,
+                                                                rel, 2 * g->max_rounds);
+                            if (used == 0) break;
+                            r2 += used;
                         }
                         r_hi = r2 > r_hi ? r2 : r_hi;
                         SSSP_HIP(hipEventRecord(g->ev_b, s));
@@ -2236,7 +2501,8 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
         restore();
         if (over) break;
         charge();
-        if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &st, verified, relaxed, bm, bc))
+        if (!step_round<WT>(g, r, SSSP_SCHED_HEURISTIC, tick0, &st, verified, relaxed,
+                            2 * g->max_rounds, bm, bc))
             break;
         if (!verify_round && row < cap) {
             for (int i = 0; i < kIn; i++) feats[(size_t)row * kIn + i] = f[i];

@@ -58,6 +58,11 @@ struct Ctrl {
     int32_t dec_seq;          // host mirror: written last by the decision kernel (round
                               // sequence number), after the fields above
     int32_t open_done;        // workgroups of k_round_open done sampling (the last one decides)
+    int32_t tail_stop;        // first round of the current run of tail rounds that did not
+                              // run (-1: all so far ran)
+    int32_t tail_rounds;      // tail rounds of the current run that ran
+    int64_t tail_relaxed;     // edge relaxations evaluated by them
+    unsigned long long tail_list;  // entries << 32 | edges listed by the tail round in flight
 };
 
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
@@ -256,6 +261,11 @@ struct sssp_graph {
     double dense_unreached = 0.5;  // ... and so do those with this sampled unreached fraction
     int dense_kernel = 0;  // dense rounds: 0 = destination-blocked pass, 1 = pass + compaction
     int plain_kernel = 0;  // plain passes: 0 = by tile count, 1 = edge-list pass, 2 = blocked pass
+    int64_t tail_edges = (int64_t)1 << 23;  // heuristic/learned: a push round whose frontier has at
+                                            // most this many out-edges starts the tail rounds (0 = off)
+    int tail_group = 4;    // tail rounds queued per host check
+    int tail_blocks = 0;   // workgroups of a tail round's relaxation (0 = 2 per CU)
+    bool tail_armed = false;  // this solve's frontier has had more than tail_edges out-edges
     bool last_round_dense = false;  // the last round ran dense (traces)
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds

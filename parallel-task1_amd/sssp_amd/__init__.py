@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsssp_mi355x.so")
 OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_INTERNAL, ERR_IO, ERR_NOT_CONVERGED = range(7)
 W_F32, W_U32, W_F64 = 0, 1, 2
 SCHED_PLAIN, SCHED_HEURISTIC, SCHED_LEARNED, SCHED_PUSH, SCHED_ALL, SCHED_PULL = range(6)
-MODE_PUSH, MODE_ALL, MODE_PULL, MODE_VERIFY, MODE_DENSE = 0, 1, 2, 3, 4
+MODE_PUSH, MODE_ALL, MODE_PULL, MODE_VERIFY, MODE_DENSE, MODE_TAIL = 0, 1, 2, 3, 4, 5
 INPUT_DEVICE, NO_CSC, NO_RELABEL, RELABEL = 1, 2, 4, 8
 MLP_IN, MLP_HID, MLP_MODES, MLP_CHUNKS = 5, 64, 3, 4
 MLP_OUT = MLP_MODES + MLP_CHUNKS

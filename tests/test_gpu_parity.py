@@ -206,6 +206,57 @@ def test_verify_pass_and_trace(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tail_edges,group", [(1 << 14, 4), (1 << 14, 1), (64, 4), (64, 3)])
+def test_tail_rounds_bit_exact(gpu, oracle, tail_edges, group):
+    """Tail rounds (push rounds queued without a decision or host round trip
+    between them, entry lists built in one pass) land on the oracle's
+    fixpoint: f32 R-MAT (several sources, heuristic and learned), u32 ER and
+    BFS levels. A bound of 64 out-edges makes the frontier cross it again and
+    again (tail runs that stop and resume); group 1 runs under trace_events
+    (one round per host check, each timed)."""
+    src, dst, w = S.rmat(16, 16, seed=13, threads=8)
+    V = 1 << 16
+    sources = [0] + list(S.pick_sources(src, V, 131, 3))
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
+        g.set_param("tail_edges", tail_edges)
+        g.set_param("tail_group", group)
+        g.set_param("trace_events", 1 if group == 1 else 0)
+        tails = 0
+        for s in sources:
+            for sched in ("heuristic", "learned"):
+                print(f"tail rounds: rmat16 f32 source {s} {sched} tail_edges {tail_edges} "
+                      f"group {group}", flush=True)
+                d, st = g.solve(int(s), sched)
+                assert np.array_equal(bits(d), bits(oracle.dijkstra(src, dst, w, V, int(s)))), \
+                    (s, sched)
+                tr = g.trace()
+                assert len(tr["mode"]) == st["iterations"]
+                assert tr["mode"][-1] == S.MODE_VERIFY
+                k = tr["mode"] == S.MODE_TAIL
+                tails += int(k.sum())
+                if group == 1 and k.any():
+                    assert (tr["frontier"][k] > 0).all() and (tr["ms"][k] > 0).all()
+                    assert (tr["frontier_edges"][k] <= tail_edges).all()
+        assert tails > 0, "no tail rounds ran"
+        ok, counts, par = g.certify(int(sources[-1]))
+        assert ok, counts
+        print(f"tail rounds: rmat16 BFS tail_edges {tail_edges} group {group}", flush=True)
+        vis, lev, st = g.bfs(int(sources[1]), "heuristic")
+        rvis, rlev, _ = oracle.bfs(src, dst, V, int(sources[1]))
+        assert np.array_equal(lev, rlev) and np.array_equal(vis.view(np.uint8), rvis)
+    V = 1 << 15
+    src, dst, w = S.er(V, 8, seed=17, weight_mode=1)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)
+        g.set_param("tail_edges", tail_edges)
+        g.set_param("tail_group", group)
+        print(f"tail rounds: er15 u32 tail_edges {tail_edges} group {group}", flush=True)
+        d, st = g.solve(5, "heuristic")
+        assert np.array_equal(d, oracle.dijkstra(src, dst, w, V, 5))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("scale,ef,seed,threads", [(12, 16, 3, 64), (16, 16, 0, 1), (18, 8, 11, 7)])
 def test_device_rmat_generator_matches_host(gpu, scale, ef, seed, threads):
     s, d, w = S.rmat(scale, ef, seed=seed, threads=threads)
