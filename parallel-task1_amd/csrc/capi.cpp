@@ -45,7 +45,7 @@ void graph_free(sssp_graph *g)
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
                     g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap,
-                    g->b_pk, g->b_cbase, g->b_wide, g->b_w, g->b_tile, g->m_rec, g->m_tile, g->mdist,
+                    g->b_pk, g->b_cbase, g->b_wide, g->b_w, g->b_tile, g->b_app, g->m_rec, g->m_tile, g->mdist,
                     g->vperm, g->vinv, g->dout != g->dist ? g->dout : nullptr, g->msrc, g->mflags};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (auto &pair : g->mexec)
@@ -217,7 +217,7 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     const int64_t Ei = g->E, Vi = g->V;
     const int64_t xe = g->xgoff_host.empty() ? 0 : g->xgoff_host.back();
     g->device_bytes = Ei * 12 + xe * 12 + g->b_edges * 8 + g->b_edges / kBlkChunk * 4 +
-                      g->b_nwide * kBlkChunk * 4 + (int64_t)g->n_btile * 24 +
+                      g->b_nwide * kBlkChunk * 4 + (int64_t)g->n_btile * (24 + kBlkThreads * 2) +
                       (Vi + 1) * 8 + (g->has_csc ? Ei * 8 + (Vi + 1) * 8 : 0) + Vi * 4 * 7 +
                       (Vi + 1) * 4 * 3 + (Vi + 1) * 8 * 2 + Vi * 24 + (g->relabel ? Vi * 12 : 0);
 }

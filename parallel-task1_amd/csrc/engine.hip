@@ -75,6 +75,7 @@ constexpr int kStageCap = 512;  // per-wave LDS staging slots for frontier appen
 struct RoundAcc {
     int64_t relaxed = 0, qedges = 0;
     int lowered = 0;
+    int appended = 0;  // frontier entries not counted in the slot otherwise (dense rounds)
     uint32_t minw = 0xFFFFFFFFu;
 };
 
@@ -82,32 +83,35 @@ template <int NW>
 __device__ __forceinline__ void flush_acc_n(RoundAcc &acc, RoundSlot *s)
 {
     __shared__ int64_t r_rel[NW], r_qe[NW];
-    __shared__ int r_low[NW];
+    __shared__ int r_low[NW], r_app[NW];
     __shared__ uint32_t r_min[NW];
     const int lane = __lane_id(), w = threadIdx.x >> 6;
     const int64_t rel = wave_sum64(acc.relaxed), qe = wave_sum64(acc.qedges);
-    const int low = wave_sum32(acc.lowered);
+    const int low = wave_sum32(acc.lowered), app = wave_sum32(acc.appended);
     const uint32_t mn = wave_min_u32(acc.minw);
     if (lane == 0) {
         r_rel[w] = rel;
         r_qe[w] = qe;
         r_low[w] = low;
+        r_app[w] = app;
         r_min[w] = mn;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int64_t a = 0, b = 0;
-        int c = 0;
+        int c = 0, q = 0;
         uint32_t m = 0xFFFFFFFFu;
         for (int i = 0; i < NW; i++) {
             a += r_rel[i];
             b += r_qe[i];
             c += r_low[i];
+            q += r_app[i];
             m = r_min[i] < m ? r_min[i] : m;
         }
         if (a) atomicAdd((unsigned long long *)&s->relaxed, (unsigned long long)a);
         if (b) atomicAdd((unsigned long long *)&s->qedges, (unsigned long long)b);
         if (c) atomicAdd(&s->lowered, c);
+        if (q) atomicAdd(&s->qlen, q);
         if (m != 0xFFFFFFFFu) atomicMin(&s->minw, m);
     }
 }
@@ -1326,8 +1330,8 @@ __global__ __launch_bounds__(kBlock) void k_lowered_compact(const uint32_t *dist
     flush_acc(acc, slot);
 }
 
-constexpr int kBlkThreads = 1024;
 constexpr int kBlkPer = (1 << kBlkBits) / kBlkThreads;  // block vertices per thread
+static_assert(kBlkPer <= 16, "a thread's appends of one tile fit a 16-bit mask");
 
 // One tile of the destination-blocked edge list, shared by the dense round
 // and the plain blocked pass: loads the tile's block of 2^kBlkBits distances
@@ -1381,11 +1385,18 @@ __device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t, const BlkLis
         atomicMin(&s_d[loc], fin ? WT::add(du, ww) : INF);
     };
     // tiles hold whole chunks and the wave's 64 lanes cover one chunk per
-    // step, so the loop bound and the chunk index are wave-uniform
-    for (int64_t g = (t.e0 >> 2) + threadIdx.x; g < (t.e1 >> 2); g += kBlkThreads) {
-        const int32_t cb = L.cbase[g >> 6];
-        const v4u pv = __builtin_nontemporal_load(&p4[g]);
-        const v4u wv = __builtin_nontemporal_load(&w4[g]);
+    // step, so the loop bound and the chunk index are wave-uniform. The next
+    // step's stream loads are issued before this step's gathers.
+    const int64_t gend = t.e1 >> 2;
+    int64_t g = (t.e0 >> 2) + threadIdx.x;
+    int32_t cb = 0;
+    v4u pv = {0u, 0u, 0u, 0u}, wv = {0u, 0u, 0u, 0u};
+    if (g < gend) {
+        cb = L.cbase[g >> 6];
+        pv = __builtin_nontemporal_load(&p4[g]);
+        wv = __builtin_nontemporal_load(&w4[g]);
+    }
+    for (; g < gend; g += kBlkThreads) {
         int32_t s0, s1, s2, s3;
         if (cb >= 0) {
             const int32_t a0 = (int32_t)(pv.x >> kBlkBits);
@@ -1414,111 +1425,221 @@ __device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t, const BlkLis
         // the src-sorted runs, hit the same line); a reuse test between them
         // would serialise them behind one another's latency
         const uint32_t u0 = dist[s0], u1 = dist[s1], u2 = dist[s2], u3 = dist[s3];
+        const int64_t gn = g + kBlkThreads;
+        int32_t cbn = 0;
+        v4u pvn = {0u, 0u, 0u, 0u}, wvn = {0u, 0u, 0u, 0u};
+        if (gn < gend) {
+            cbn = L.cbase[gn >> 6];
+            pvn = __builtin_nontemporal_load(&p4[gn]);
+            wvn = __builtin_nontemporal_load(&w4[gn]);
+        }
         relax(u0, pv.x & kOffMask, wv.x);
         relax(u1, pv.y & kOffMask, wv.y);
         relax(u2, pv.z & kOffMask, wv.z);
         relax(u3, pv.w & kOffMask, wv.w);
+        cb = cbn;
+        pv = pvn;
+        wv = wvn;
     }
     __syncthreads();
     return relaxed;
 }
 
+// Round counters of a kBlkThreads workgroup through LDS (buf: 5 * kBlkThreads
+// words no longer in use): every thread stores its partials, then the first
+// wave sums them. The shuffle reductions of flush_acc_n, run by every wave
+// with the write-back's values live, pushed k_blk_pass past its 64 registers
+// (20 spilled); this keeps it at 57, none spilled.
+__device__ __forceinline__ void flush_acc_lds(const RoundAcc &acc, RoundSlot *s, uint32_t *buf)
+{
+    constexpr int N = kBlkThreads;
+    const int tid = threadIdx.x;
+    __syncthreads();  // buf's last readers are done
+    buf[0 * N + tid] = (uint32_t)acc.relaxed;  // per thread < 2^32: a tile has fewer edges
+    buf[1 * N + tid] = (uint32_t)acc.qedges;   // at most 16 degrees, E < 2^31
+    buf[2 * N + tid] = (uint32_t)acc.lowered;
+    buf[3 * N + tid] = (uint32_t)acc.appended;
+    buf[4 * N + tid] = acc.minw;
+    __syncthreads();
+    if (tid < 64) {
+        unsigned long long a = 0, b = 0;
+        uint32_t c = 0, q = 0, m = 0xFFFFFFFFu;
+        for (int i = tid; i < N; i += 64) {
+            a += buf[i];
+            b += buf[N + i];
+            c += buf[2 * N + i];
+            q += buf[3 * N + i];
+            m = min(m, buf[4 * N + i]);
+        }
+        a = wave_sum64((long long)a);
+        b = wave_sum64((long long)b);
+        c = (uint32_t)wave_sum32((int)c);
+        q = (uint32_t)wave_sum32((int)q);
+        m = wave_min_u32(m);
+        if (tid == 0) {
+            if (a) atomicAdd((unsigned long long *)&s->relaxed, a);
+            if (b) atomicAdd((unsigned long long *)&s->qedges, b);
+            if (c) atomicAdd(&s->lowered, (int)c);
+            if (q) atomicAdd(&s->qlen, (int)q);
+            if (m != 0xFFFFFFFFu) atomicMin(&s->minw, m);
+        }
+    }
+}
+
 // Dense all-edge round over the destination-blocked edge list. A workgroup
 // takes one tile (blk_relax_tile), then writes back the vertices it lowered
-// and appends them to the next frontier with the round's stamp. A block with
-// one tile belongs to one workgroup: plain stores, plain stamps, no global
-// atomics in the whole round. Tiles of a split block (a high in-degree
-// block) write back with a returning atomicMin and stamp with atomicExch, so
-// each lowered vertex is appended once. The workgroup reserves queue space
-// for all its appends with one atomicAdd: the append counts per (vertex row
-// k, wave) are scanned in LDS, and the vertices are written in slot order.
-// (Reserving per wave and flush, as the edge kernels do, kept every wave
-// waiting on a returning global atomic up to 16 times per tile: 0.3 ms of a
-// heavy round at R-MAT scale 25.) Fills the round's counters.
+// and marks them with the round's stamp. A block with one tile belongs to
+// one workgroup: plain stores, plain stamps, no global atomics in the whole
+// round. Tiles of a split block (a high in-degree block) write back with a
+// returning atomicMin and stamp with atomicExch, so each lowered vertex is
+// appended once. The next frontier is left as one 16-bit mask per thread and
+// tile (bit k: the thread's vertex of row k), counted into the slot; most
+// dense rounds are followed by another, which reads no frontier, and a push
+// round first builds its queue from the masks (k_frontier_from_bits). Writing
+// the queue here cost 0.06-0.07 ms of a 1.2 ms heavy round at R-MAT scale 25:
+// the workgroup had to wait for its returning queue reservation (one
+// atomicAdd after a scan of its append counts) before the writes. Fills the
+// round's counters.
 template <class WT>
 __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_blk_pass(
     const BlkTile *__restrict__ tiles, const BlkList L, uint32_t *dist, int64_t V,
-    int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, int32_t *qnext, RoundSlot *slot)
+    int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, uint16_t *app_bits, RoundSlot *slot)
 {
-    constexpr int NW = kBlkThreads / 64;
-    static_assert(kBlkPer * NW == 256, "the append scan runs 4 counts per lane of one wave");
     __shared__ uint32_t s_d[1 << kBlkBits];
-    __shared__ int32_t s_cnt[kBlkPer * NW];  // appends per (row k, wave), then their offsets
-    __shared__ int32_t s_qbase;
+    static_assert((1 << kBlkBits) >= 5 * kBlkThreads, "the counters' LDS fits the block's");
     RoundAcc acc;
-    const int lane = __lane_id(), w = threadIdx.x >> 6;
     const BlkTile t = tiles[blockIdx.x];
     const int64_t base = (int64_t)t.blk << kBlkBits;
     const int nb = (int)min((int64_t)(1 << kBlkBits), V - base);
     acc.relaxed = blk_relax_tile<WT, true>(t, L, dist, base, nb, s_d);
-    uint32_t appm = 0;  // bit k: this lane appends its vertex of row k
+    // write-back: the distance loads of the thread's 16 rows in flight together
+    uint32_t lowm = 0;  // bit k: this lane lowered its vertex of row k
 #pragma unroll
-    for (int k = 0; k < kBlkPer; k++) {  // block-uniform: every lane reaches the ballot
+    for (int k = 0; k < kBlkPer; k++) {
         const int i = k * kBlkThreads + threadIdx.x;
-        const int32_t v = (int32_t)(base + i);
-        bool app = false;
         if (i < nb) {
             // below the value in global memory: this tile lowers v (for a
             // block of one tile that value is the one loaded; for a split
             // block another tile may lower it further before the returning
             // atomicMin, which then reports no change)
+            const int32_t v = (int32_t)(base + i);
             const uint32_t nv = s_d[i];
             if (nv < dist[v]) {
                 bool low = true;
                 if (t.split) low = atomicMin(&dist[v], nv) > nv;
                 else dist[v] = nv;  // only this workgroup writes the block
                 if (low) {
+                    lowm |= 1u << k;
                     acc.lowered++;
                     acc.minw = min(acc.minw, nv);
-                    const int64_t deg = row_ptr[v + 1] - row_ptr[v];
-                    if (deg > 0) {
-                        if (t.split) {
-                            app = atomicExch(&stamp[v], tick_cur) != tick_cur;
-                        } else {
-                            stamp[v] = tick_cur;
-                            app = true;
-                        }
-                        if (app) acc.qedges += deg;
-                    }
                 }
             }
         }
-        appm |= (uint32_t)app << k;
-        const uint64_t m = __ballot(app);
-        if (lane == 0) s_cnt[k * NW + w] = __popcll(m);
+    }
+    // appends: out-degrees and stamps of the lowered vertices, four rows'
+    // loads at a time (all sixteen held 64-bit offsets past the register
+    // budget of two workgroups per CU, and spilled)
+    uint32_t appm = 0;  // bit k: this lane appends its vertex of row k
+#pragma unroll
+    for (int k0 = 0; k0 < kBlkPer; k0 += 4) {
+        if (((lowm >> k0) & 15u) == 0) continue;
+        int64_t r0[4], r1[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            r0[j] = r1[j] = 0;
+            if ((lowm >> (k0 + j)) & 1u) {
+                const int64_t v = base + (int64_t)(k0 + j) * kBlkThreads + threadIdx.x;
+                r0[j] = row_ptr[v];
+                r1[j] = row_ptr[v + 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t deg = r1[j] - r0[j];
+            if (deg > 0) {
+                const int32_t v = (int32_t)(base + (int64_t)(k0 + j) * kBlkThreads + threadIdx.x);
+                bool app;
+                if (t.split) {
+                    app = atomicExch(&stamp[v], tick_cur) != tick_cur;
+                } else {
+                    stamp[v] = tick_cur;
+                    app = true;
+                }
+                if (app) {
+                    acc.qedges += deg;
+                    appm |= 1u << (k0 + j);
+                }
+            }
+        }
+    }
+    // the frontier stays in these masks (one store per thread, every tile,
+    // so no stale bits survive) until a round that reads a queue needs it:
+    // k_frontier_from_bits
+    app_bits[(int64_t)blockIdx.x * kBlkThreads + threadIdx.x] = (uint16_t)appm;
+    acc.appended = __popc(appm);
+    flush_acc_lds(acc, slot, s_d);
+}
+
+// The queue of the frontier a dense round left in its masks: a workgroup
+// takes kMaskTiles consecutive tiles (their masks stay in registers, 4 per
+// thread per tile), reserves space for all their vertices with one atomicAdd
+// and writes them. The order (tile, thread, row) differs from an appended
+// queue's; no round depends on the order.
+constexpr int kMaskTiles = 8;
+static_assert(kBlkThreads == 4 * kBlock, "a thread reads 4 masks of a tile as one 64-bit word");
+
+__global__ __launch_bounds__(kBlock) void k_frontier_from_bits(const BlkTile *__restrict__ tiles,
+                                                               int32_t n_tiles,
+                                                               const uint16_t *__restrict__ bits,
+                                                               int32_t *q, int32_t *len)
+{
+    __shared__ int s_w[kBlock / 64];
+    __shared__ int s_base;
+    const int tid = threadIdx.x, lane = __lane_id(), w = tid >> 6;
+    const int64_t t0 = (int64_t)blockIdx.x * kMaskTiles;
+    uint64_t word[kMaskTiles];
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < kMaskTiles; i++) {
+        word[i] = 0;
+        if (t0 + i < n_tiles)
+            word[i] = reinterpret_cast<const uint64_t *>(bits)[(t0 + i) * kBlock + tid];
+        cnt += __popcll(word[i]);
+    }
+    // exclusive scan of the counts over the workgroup
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    if (tid == 0) {
+        int tot = 0;
+        for (int i = 0; i < kBlock / 64; i++) {
+            const int c = s_w[i];
+            s_w[i] = tot;
+            tot += c;
+        }
+        s_base = tot > 0 ? atomicAdd(len, tot) : 0;
     }
     __syncthreads();
-    if (w == 0) {
-        // exclusive scan of the 256 counts in (k, wave) order: 4 per lane
-        const int c0 = s_cnt[4 * lane], c1 = s_cnt[4 * lane + 1], c2 = s_cnt[4 * lane + 2],
-                  c3 = s_cnt[4 * lane + 3];
-        const int sum = c0 + c1 + c2 + c3;
-        int incl = sum;
+    int pos = s_base + s_w[w] + inc - cnt;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o);
-            if (lane >= o) incl += y;
-        }
-        const int ex = incl - sum;
-        s_cnt[4 * lane] = ex;
-        s_cnt[4 * lane + 1] = ex + c0;
-        s_cnt[4 * lane + 2] = ex + c0 + c1;
-        s_cnt[4 * lane + 3] = ex + c0 + c1 + c2;
-        if (lane == 63) s_qbase = incl > 0 ? atomicAdd(&slot->qlen, incl) : 0;
-    }
-    __syncthreads();
-    if (appm) {
-        const int32_t qb = s_qbase;
-#pragma unroll
-        for (int k = 0; k < kBlkPer; k++) {
-            const bool app = (appm >> k) & 1u;
-            const uint64_t m = __ballot(app);
-            if (app)
-                qnext[qb + s_cnt[k * NW + w] + __popcll(m & lanemask_lt())] =
-                    (int32_t)(base + k * kBlkThreads + threadIdx.x);
+    for (int i = 0; i < kMaskTiles; i++) {
+        uint64_t m = word[i];
+        if (m == 0) continue;
+        const int64_t base = (int64_t)tiles[t0 + i].blk << kBlkBits;
+        while (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            // mask j = 4 tid + b / 16 of the tile (thread j of k_blk_pass),
+            // bit b % 16 = its row
+            const int j = 4 * tid + (b >> 4), k = b & 15;
+            q[pos++] = (int32_t)(base + (int64_t)k * kBlkThreads + j);
         }
     }
-    flush_acc_n<NW>(acc, slot);
 }
 
 // Plain Bellman-Ford pass (bellman_ford.cu:62-99) over the destination-blocked
@@ -1760,6 +1881,21 @@ AdvArgs base_args(sssp_graph *g, int32_t r, int32_t tick0)
     return a;
 }
 
+// Queues the frontier of round r when a dense round left it in its append
+// masks (frontier_bits).
+void frontier_to_queue(sssp_graph *g, int32_t r)
+{
+    if (!g->frontier_bits) return;
+    hipStream_t s = g->stream;
+    k_fill<int32_t><<<1, 64, 0, s>>>(&g->ctrl->mat_n, 1, 0);
+    SSSP_LAUNCH_CHECK("k_fill mat_n");
+    const unsigned grid = (unsigned)((g->n_btile + kMaskTiles - 1) / kMaskTiles);
+    k_frontier_from_bits<<<grid, kBlock, 0, s>>>(g->b_tile, g->n_btile, g->b_app,
+                                                 g->queue[r & 1], &g->ctrl->mat_n);
+    SSSP_LAUNCH_CHECK("k_frontier_from_bits");
+    g->frontier_bits = false;
+}
+
 // Runs one relaxation round in the given mode. `pile` is the pile buffer the
 // round reads; the host mirror of the control block holds its counters.
 // Returns true when a push round wrote the next pile (the caller swaps).
@@ -1775,6 +1911,9 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
     const bool snap_ok = r > 0 && g->snap_round == r - 1;
     g->snap_round = -1;  // any round but a dense one changes distances behind the snapshot
     g->last_round_dense = false;
+    // a round that reads the frontier as a queue builds it first (push);
+    // every round but k_blk_pass's leaves the next frontier queued
+    if (mode != SSSP_MODE_PUSH || verify) g->frontier_bits = false;
     if (mode != SSSP_MODE_PUSH && !verify && pn > 0) {
         k_pile_flush<<<grid_for(pn, 4096), kBlock, 0, s>>>(g->pile[p], pn, g->stamp, a.tick_prev,
                                                            g->farst, g->ctrl, p);
@@ -1797,8 +1936,9 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
             if (g->dense_kernel == 0 && g->n_btile > 0) {
                 k_blk_pass<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, s>>>(
                     g->b_tile, blk_list(g), g->dist, g->V, g->stamp, a.tick_cur,
-                    g->row_ptr, a.qnext, a.slot);
+                    g->row_ptr, g->b_app, a.slot);
                 SSSP_LAUNCH_CHECK("k_blk_pass");
+                g->frontier_bits = true;
                 return false;
             }
             // the snapshot still holds the distances a dense round r - 1 left
@@ -1834,6 +1974,7 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
     }
     switch (mode) {
     case SSSP_MODE_PUSH: {
+        frontier_to_queue(g, r);
         int32_t *q = g->queue[r & 1];
         const bool piled = g->delta > 0;
         NearRule rule{0xFFFFFFFFu, 0, nullptr, a.tick_cur};
@@ -2157,6 +2298,7 @@ int32_t step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stat
         g->tail_armed && (sched == SSSP_SCHED_HEURISTIC || sched == SSSP_SCHED_LEARNED) &&
         piled == 0 && g->delta <= 0.0 && cur.qlen > 0 && cur.qedges <= g->tail_edges &&
         r + 1 < r_cap) {
+        frontier_to_queue(g, r);
         const int32_t r1 = run_tail<WT>(g, r, tick0, r_cap, st, relaxed);
         if (r1 > r) return r1 - r;
         // (the first tail round always runs: its frontier was checked here)
@@ -2193,6 +2335,7 @@ int32_t solve_rounds(sssp_graph *g, int sched, int32_t tick0, sssp_stats *st)
     int64_t relaxed = 0;
     g->pile_cur = 0;
     g->tail_armed = false;
+    g->frontier_bits = false;
     int32_t r = 0;
     for (;;) {
         if (r >= g->max_rounds)
@@ -2402,6 +2545,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
     sssp_stats st{};
     g->collect_rounds = 0;
     g->tail_armed = false;
+    g->frontier_bits = false;
     auto charge = [&] {
         if (++g->collect_rounds > g->collect_budget) {
             SSSP_HIP(hipStreamSynchronize(s));
@@ -2422,6 +2566,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
             throw Error{SSSP_ERR_NOT_CONVERGED, "collect: too many rounds"};
         }
         const int p = g->pile_cur;
+        frontier_to_queue(g, r);  // the saved state holds the frontier as a queue
         copy(sd, g->dist, 4 * V);
         copy(sst, g->stamp, 4 * V);
         copy(sfa, g->farst, 4 * V);
@@ -2432,6 +2577,7 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
         auto restore = [&] {
             g->snap_round = -1;
             g->tail_armed = armed;
+            g->frontier_bits = false;  // queued before the state was saved
             copy(g->dist, sd, 4 * V);
             copy(g->stamp, sst, 4 * V);
             copy(g->farst, sfa, 4 * V);

@@ -843,8 +843,12 @@ void retile_blocked(sssp_graph *g, int64_t cap, int order, int range_bits)
     SSSP_HIP(hipMemcpyAsync(td, tiles.data(), sizeof(BlkTile) * tiles.size(), hipMemcpyHostToDevice,
                             g->stream));
     SSSP_HIP(hipStreamSynchronize(g->stream));
+    uint16_t *ad = dmalloc<uint16_t>(std::max<size_t>(tiles.size(), 1) * kBlkThreads);
     dfree(g->b_tile);
+    dfree(g->b_app);
     g->b_tile = td;
+    g->b_app = ad;
+    g->frontier_bits = false;
     g->n_btile = (int32_t)tiles.size();
     g->blk_tile_cap = cap;
     g->blk_tile_order = order;
@@ -1106,6 +1110,7 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         g->n_btile = (int32_t)tiles.size();
         g->b_edges = Ep;
         g->b_tile = dmalloc<BlkTile>(tiles.size());
+        g->b_app = dmalloc<uint16_t>(tiles.size() * kBlkThreads);
         SSSP_HIP(hipMemcpyAsync(g->b_tile, tiles.data(), sizeof(BlkTile) * tiles.size(),
                                 hipMemcpyHostToDevice, s));
         k_blk_gather<<<grid_for(E), kBlock, 0, s>>>(bkeys, bperm, E, bptr, boff, g->el_src,

@@ -63,6 +63,7 @@ struct Ctrl {
     int32_t tail_rounds;      // tail rounds of the current run that ran
     int64_t tail_relaxed;     // edge relaxations evaluated by them
     unsigned long long tail_list;  // entries << 32 | edges listed by the tail round in flight
+    int32_t mat_n;            // vertices written by k_frontier_from_bits
 };
 
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
@@ -75,6 +76,7 @@ struct Ctrl {
 // while the graph holds kBlkTileCount tiles' worth of edges or more);
 // `split` = the block has more than one tile.
 constexpr int kBlkBits = 14;
+constexpr int kBlkThreads = 1024;  // threads of a blocked-pass workgroup
 static_assert(kBlkBits <= 16, "block offsets fit the low half of a packed edge");
 constexpr int kBlkDeltaBits = 32 - kBlkBits;  // source delta bits of a packed edge
 constexpr int64_t kBlkChunk = 256;  // edge slots per chunk: 4 per lane of one wave
@@ -162,6 +164,8 @@ struct sssp_graph {
     uint32_t *b_w = nullptr;
     sssp::BlkTile *b_tile = nullptr;
     int32_t n_btile = 0;
+    uint16_t *b_app = nullptr;  // dense rounds' append masks: kBlkThreads per tile
+    bool frontier_bits = false;  // the coming round's frontier is in b_app, not yet queued
     std::vector<int64_t> b_off_host;  // padded block offsets (tiles are rebuilt from them)
     int64_t blk_tile_cap = 0;         // edges per tile at most
     int blk_tile_order = 0;           // dispatch order of the tiles (make_blk_tiles)
