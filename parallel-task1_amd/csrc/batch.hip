@@ -382,15 +382,25 @@ __device__ __forceinline__ bool multi_tile(const BlkTile &t, const MRec *__restr
     // 32-vertex blocks 44.0 (16); 16-vertex blocks (16 KB, 8 workgroups per
     // CU) 51.4 (16), 54.7 (8); 8-vertex blocks 51.1 (16)
     constexpr int R = 8;
-    for (; e + R <= t.e1; e += R) {
-        MRec r[R];
-        uint32_t du[R];
+    MRec r[R];
+    if (e + R <= t.e1) {
 #pragma unroll
         for (int k = 0; k < R; k++) r[k] = rec[e + k];
+    }
+    for (; e + R <= t.e1; e += R) {
+        uint32_t du[R];
 #pragma unroll
         for (int k = 0; k < R; k++) du[k] = act ? D[(int64_t)r[k].src * n + s] : INF;
+        // the next step's records load under this step's gathers
+        MRec rn[R];
+        if (e + 2 * R <= t.e1) {
+#pragma unroll
+            for (int k = 0; k < R; k++) rn[k] = rec[e + R + k];
+        }
 #pragma unroll
         for (int k = 0; k < R; k++) relax(r[k], du[k]);
+#pragma unroll
+        for (int k = 0; k < R; k++) r[k] = rn[k];
     }
     for (; e < t.e1; e++) {
         const MRec r = rec[e];
