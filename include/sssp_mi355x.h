@@ -186,7 +186,7 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
  * "batch_capture", "decide_spin", "blk_tile_cap", "blk_tile_order",
  * "blk_range_bits", "plain_kernel", "tail_edges", "tail_group", "tail_blocks",
- * "spec_dense", "trace_events"; "cu_count",
+ * "trace_events"; "cu_count",
  * "collect_rounds", "multi_passes", "blk_tiles", "blk_wide_chunks" and
  * "plain_blocked" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
@@ -222,10 +222,6 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * is empty or has more out-edges than tail_edges again; 0 turns them off.
  * Traces show them as SSSP_MODE_TAIL, with sizes and times only under
  * trace_events (which queues them one at a time).
- * spec_dense (default 1): a round after a dense round, or after tail rounds
- * that emptied the frontier, is queued before its decision and runs only if
- * the decision finds it dense (off under trace_events, delta > 0 or
- * dense_kernel 1); 0 always waits for the decision.
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with
  * d >= (min distance over the frontier and pile) + delta and out-degree >=
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.

@@ -624,9 +624,6 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "tail_group") {
             SSSP_REQUIRE(value >= 1 && value <= 64, "tail_group must be in [1, 64]");
             g->tail_group = (int)value;
-        } else if (n == "spec_dense") {
-            SSSP_REQUIRE(value == 0 || value == 1, "spec_dense must be 0 or 1");
-            g->spec_dense = (int)value;
         } else if (n == "tail_blocks") {
             SSSP_REQUIRE(value >= 0 && value <= 65536, "tail_blocks must be in [0, 65536]");
             g->tail_blocks = (int)value;
@@ -666,7 +663,6 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "tail_edges") *value = (double)g->tail_edges;
         else if (n == "tail_group") *value = g->tail_group;
         else if (n == "tail_blocks") *value = g->tail_blocks;
-        else if (n == "spec_dense") *value = g->spec_dense;
         else if (n == "multi_passes") *value = g->multi_passes;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;

@@ -930,12 +930,6 @@ struct DecideArgs {
     int pile;          // pile buffer the round reads
     Ctrl *mirror;      // device address of the host mirror (coherent host memory)
     int32_t seq;       // written to mirror->dec_seq once the mirror holds this round
-    // speculative dense round: the host queued k_blk_pass behind this kernel,
-    // which runs it only if the round turns out dense (spec_seq = seq)
-    int spec;
-    int fixed_mode;       // the schedule's mode when it has one (push/all/pull), else -1
-    int verify_pending;   // a round with nothing pending runs the fixpoint check
-    double dense_all_frac, dense_unreached;
 };
 
 // Features, the hand-written rule and (learned schedule) the MLP, computed
@@ -1025,17 +1019,6 @@ __device__ __forceinline__ void decide_round(Ctrl *c, const DecideArgs &d)
         c->dec_chunk = chunk;
         c->dec_mlp = by_mlp;
         c->dec_heur_mode = hmode;
-        // whether the round runs dense, by the host's rule (step_round and
-        // run_round), for the speculative k_blk_pass already queued
-        int rm = d.fixed_mode >= 0 ? d.fixed_mode : mode;
-        if (rm == SSSP_MODE_PULL && !d.has_csc) rm = SSSP_MODE_ALL;
-        const bool over = cur->qlen == 0 && c->pile_n[d.pile] == 0;
-        const bool verify = over && d.verify_pending;
-        if (!verify && cur->qlen == 0 && rm != SSSP_MODE_PUSH) rm = SSSP_MODE_PUSH;
-        const double unr = cur->sample_n > 0 ? (double)cur->sample_unreached / cur->sample_n : 0.0;
-        const bool dense = verify || (!over && rm == SSSP_MODE_ALL &&
-                                      (fe >= d.dense_all_frac * d.E || unr >= d.dense_unreached));
-        c->spec_seq = (d.spec && dense) ? d.seq : 0;
         for (int i = 0; i < kIn; i++) c->feat[i] = s_f[i];
         for (int i = 0; i < kOut; i++) c->logits[i] = by_mlp ? s_o[i] : 0.f;
         // what the host reads to launch the round, written straight into
@@ -1050,7 +1033,6 @@ __device__ __forceinline__ void decide_round(Ctrl *c, const DecideArgs &d)
         m->dec_chunk = chunk;
         m->dec_mlp = by_mlp;
         m->dec_heur_mode = hmode;
-        m->spec_seq = c->spec_seq;
         for (int i = 0; i < kIn; i++) m->feat[i] = s_f[i];
         for (int i = 0; i < kOut; i++) m->logits[i] = by_mlp ? s_o[i] : 0.f;
         // the host polls dec_seq instead of waiting for the stream: every
@@ -1384,10 +1366,19 @@ __device__ __forceinline__ int64_t blk_relax_tile(const BlkTile &t, const BlkLis
     constexpr uint32_t INF = WT::INF;
     constexpr uint32_t kOffMask = (1u << kBlkBits) - 1;
     static_assert(kBlkChunk == 4 * 64, "a chunk is 4 slots per lane of one wave");
+    // the block's distances: all rows' loads in flight before the first LDS
+    // store (a load per row under its bounds test made the compiler wait for
+    // each row in turn: 16 memory round trips per tile)
+    uint32_t row[kBlkPer];
 #pragma unroll
     for (int k = 0; k < kBlkPer; k++) {
         const int i = k * kBlkThreads + threadIdx.x;
-        s_d[i] = i < nb ? dist[base + i] : INF;
+        row[k] = dist[base + min(i, nb - 1)];  // nb >= 1: an in-bounds read
+    }
+#pragma unroll
+    for (int k = 0; k < kBlkPer; k++) {
+        const int i = k * kBlkThreads + threadIdx.x;
+        s_d[i] = i < nb ? row[k] : INF;
     }
     __syncthreads();
     const v4u *p4 = reinterpret_cast<const v4u *>(L.pk);
@@ -1521,12 +1512,8 @@ __device__ __forceinline__ void flush_acc_lds(const RoundAcc &acc, RoundSlot *s,
 template <class WT>
 __global__ __launch_bounds__(kBlkThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_blk_pass(
     const BlkTile *__restrict__ tiles, const BlkList L, uint32_t *dist, int64_t V,
-    int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, uint16_t *app_bits, RoundSlot *slot,
-    const int32_t *spec, int32_t seq)
+    int32_t *stamp, int32_t tick_cur, const int64_t *row_ptr, uint16_t *app_bits, RoundSlot *slot)
 {
-    // launched ahead of its round's decision: runs only if the decision
-    // kernel found the round dense (grid-uniform)
-    if (spec && *spec != seq) return;
     __shared__ uint32_t s_d[1 << kBlkBits];
     static_assert((1 << kBlkBits) >= 5 * kBlkThreads, "the counters' LDS fits the block's");
     RoundAcc acc;
@@ -1918,24 +1905,12 @@ void frontier_to_queue(sssp_graph *g, int32_t r)
     g->frontier_bits = false;
 }
 
-// The dense round r over the destination-blocked list; with `spec` set it is
-// queued ahead of the round's decision and runs only if that finds the round
-// dense (the decision kernel sets *spec = seq).
-template <class WT>
-void launch_dense(sssp_graph *g, int32_t r, int32_t tick0, const int32_t *spec, int32_t seq)
-{
-    k_blk_pass<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, g->stream>>>(
-        g->b_tile, blk_list(g), g->dist, g->V, g->stamp, tick0 + r + 1, g->row_ptr, g->b_app,
-        &g->ctrl->slot[(r + 1) & 1], spec, seq);
-    SSSP_LAUNCH_CHECK("k_blk_pass");
-}
-
 // Runs one relaxation round in the given mode. `pile` is the pile buffer the
 // round reads; the host mirror of the control block holds its counters.
 // Returns true when a push round wrote the next pile (the caller swaps).
 template <class WT>
 bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, const RoundSlot &cur,
-               bool verify, bool spec_ran = false)
+               bool verify)
 {
     AdvArgs a = base_args<WT>(g, r, tick0);
     hipStream_t s = g->stream;
@@ -1948,7 +1923,6 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
     // a round that reads the frontier as a queue builds it first (push);
     // every round but k_blk_pass's leaves the next frontier queued
     if (mode != SSSP_MODE_PUSH || verify) g->frontier_bits = false;
-    g->spec_hint = false;
     if (mode != SSSP_MODE_PUSH && !verify && pn > 0) {
         k_pile_flush<<<grid_for(pn, 4096), kBlock, 0, s>>>(g->pile[p], pn, g->stamp, a.tick_prev,
                                                            g->farst, g->ctrl, p);
@@ -1969,9 +1943,11 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
             // against a snapshot
             g->last_round_dense = true;
             if (g->dense_kernel == 0 && g->n_btile > 0) {
-                if (!spec_ran) launch_dense<WT>(g, r, tick0, nullptr, 0);
+                k_blk_pass<WT><<<(unsigned)g->n_btile, kBlkThreads, 0, s>>>(
+                    g->b_tile, blk_list(g), g->dist, g->V, g->stamp, a.tick_cur,
+                    g->row_ptr, g->b_app, a.slot);
+                SSSP_LAUNCH_CHECK("k_blk_pass");
                 g->frontier_bits = true;
-                g->spec_hint = true;
                 return false;
             }
             // the snapshot still holds the distances a dense round r - 1 left
@@ -1991,7 +1967,6 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
             g->snap_round = r;
             return false;
         }
-        if (spec_ran) throw Error{SSSP_ERR_INTERNAL, "speculative dense round disagrees with the host"};
         // all-edge round streamed over the src-sorted edge list; the chunk
         // bucket sets the grid (2, 4, 8 or 16 workgroups per CU)
         const int64_t nblocks = (int64_t)g->cu_count * (2 << kidx);
@@ -2006,7 +1981,6 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
         SSSP_LAUNCH_CHECK("k_el_active");
         return false;
     }
-    if (spec_ran) throw Error{SSSP_ERR_INTERNAL, "speculative dense round disagrees with the host"};
     switch (mode) {
     case SSSP_MODE_PUSH: {
         frontier_to_queue(g, r);
@@ -2244,8 +2218,6 @@ int32_t run_tail(sssp_graph *g, int32_t r, int32_t tick0, int32_t r_cap, sssp_st
     }
     st->iterations += done;
     st->n_push += done;
-    // a run that emptied the frontier is followed by the fixpoint check
-    g->spec_hint = h.slot[r & 1].qlen == 0;
     return r;
 }
 
@@ -2280,37 +2252,18 @@ int32_t step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stat
     da.pile = g->pile_cur;
     da.mirror = g->ctrl_host_dev;
     da.seq = ++g->dec_seq;
-    // A round after a dense round (or after tail rounds that emptied the
-    // frontier: the fixpoint check) is queued before its decision: the host's
-    // round trip (its wait for the decision, then the launch) then runs while
-    // the GPU works, and the kernel returns at once if the round is not dense.
-    const bool spec = g->spec_dense && g->spec_hint && force_mode < 0 && !g->trace_events &&
-                      g->dense_kernel == 0 && g->n_btile > 0 && g->delta <= 0.0;
-    da.spec = spec ? 1 : 0;
-    da.fixed_mode = sched == SSSP_SCHED_PUSH  ? SSSP_MODE_PUSH
-                    : sched == SSSP_SCHED_ALL  ? SSSP_MODE_ALL
-                    : sched == SSSP_SCHED_PULL ? (g->has_csc ? SSSP_MODE_PULL : SSSP_MODE_ALL)
-                                               : -1;
-    da.verify_pending = (g->verify && !verified) ? 1 : 0;
-    da.dense_all_frac = g->dense_all_frac;
-    da.dense_unreached = g->dense_unreached;
     k_round_open<<<grid_for(ns, 64), kBlock, 0, s>>>(g->ctrl, r, g->pile_cur, g->sample_v,
                                                      g->has_csc ? g->sample_in : nullptr, ns,
                                                      g->dist, WT::INF, da);
     SSSP_LAUNCH_CHECK("k_round_open");
-    if (spec) launch_dense<WT>(g, r, tick0, &g->ctrl->spec_seq, da.seq);
     wait_decision(g, da.seq);
-    const bool spec_ran = spec && g->ctrl_host->spec_seq == da.seq;
     const RoundSlot cur = g->ctrl_host->slot[r & 1];
     const int32_t piled = g->ctrl_host->pile_n[g->pile_cur];
     const int64_t piled_edges = g->ctrl_host->pile_edges[g->pile_cur];
     relaxed += cur.relaxed;
     bool verify = false;
     if (cur.qlen == 0 && piled == 0) {
-        if (!g->verify || verified) {
-            if (spec_ran) throw Error{SSSP_ERR_INTERNAL, "speculative dense round after the fixpoint"};
-            return 0;
-        }
+        if (!g->verify || verified) return 0;
         verify = true;
     }
     int mode, kidx;
@@ -2354,14 +2307,13 @@ int32_t step_round(sssp_graph *g, int32_t r, int sched, int32_t tick0, sssp_stat
         g->tail_armed && (sched == SSSP_SCHED_HEURISTIC || sched == SSSP_SCHED_LEARNED) &&
         piled == 0 && g->delta <= 0.0 && cur.qlen > 0 && cur.qedges <= g->tail_edges &&
         r + 1 < r_cap) {
-        if (spec_ran) throw Error{SSSP_ERR_INTERNAL, "speculative dense round disagrees with the host"};
         frontier_to_queue(g, r);
         const int32_t r1 = run_tail<WT>(g, r, tick0, r_cap, st, relaxed);
         if (r1 > r) return r1 - r;
         // (the first tail round always runs: its frontier was checked here)
     }
     if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
-    if (run_round<WT>(g, mode, kidx, r, tick0, cur, verify, spec_ran)) g->pile_cur ^= 1;
+    if (run_round<WT>(g, mode, kidx, r, tick0, cur, verify)) g->pile_cur ^= 1;
     float ms = 0.f;
     if (g->trace_events) {
         SSSP_HIP(hipEventRecord(g->ev_r1, s));
@@ -2393,7 +2345,6 @@ int32_t solve_rounds(sssp_graph *g, int sched, int32_t tick0, sssp_stats *st)
     g->pile_cur = 0;
     g->tail_armed = false;
     g->frontier_bits = false;
-    g->spec_hint = false;
     int32_t r = 0;
     for (;;) {
         if (r >= g->max_rounds)
@@ -2604,7 +2555,6 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
     g->collect_rounds = 0;
     g->tail_armed = false;
     g->frontier_bits = false;
-    g->spec_hint = false;
     auto charge = [&] {
         if (++g->collect_rounds > g->collect_budget) {
             SSSP_HIP(hipStreamSynchronize(s));
@@ -2632,11 +2582,10 @@ void collect_t(sssp_graph *g, int32_t source, int32_t cap, float *feats, float *
         copy(sq, g->queue[r & 1], 4 * (V + 1));
         copy(sp, g->pile[p], 4 * V);
         copy(sc, g->ctrl, sizeof(Ctrl));
-        const bool armed = g->tail_armed, hint = g->spec_hint;
+        const bool armed = g->tail_armed;
         auto restore = [&] {
             g->snap_round = -1;
             g->tail_armed = armed;
-            g->spec_hint = hint;
             g->frontier_bits = false;  // queued before the state was saved
             copy(g->dist, sd, 4 * V);
             copy(g->stamp, sst, 4 * V);

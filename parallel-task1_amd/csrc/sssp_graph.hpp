@@ -64,7 +64,6 @@ struct Ctrl {
     int64_t tail_relaxed;     // edge relaxations evaluated by them
     unsigned long long tail_list;  // entries << 32 | edges listed by the tail round in flight
     int32_t mat_n;            // vertices written by k_frontier_from_bits
-    int32_t spec_seq;         // decision sequence number whose queued k_blk_pass runs (0: none)
 };
 
 // Destination-blocked edge list: edges sorted by (dst >> kBlkBits, src), each
@@ -271,8 +270,6 @@ struct sssp_graph {
     int tail_group = 4;    // tail rounds queued per host check
     int tail_blocks = 0;   // workgroups of a tail round's relaxation (0 = 2 per CU)
     bool tail_armed = false;  // this solve's frontier has had more than tail_edges out-edges
-    bool spec_hint = false;   // the coming round is likely dense: queue k_blk_pass before its decision
-    int spec_dense = 1;       // 0: never launch a dense round ahead of its decision
     bool last_round_dense = false;  // the last round ran dense (traces)
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds

@@ -257,45 +257,6 @@ def test_tail_rounds_bit_exact(gpu, oracle, tail_edges, group):
 
 
 @pytest.mark.gpu
-def test_dense_round_queued_ahead_of_decision(gpu, oracle):
-    """A round after a dense round (or after tail rounds that emptied the
-    frontier) is queued before its decision and runs only if the decision
-    kernel finds it dense, by the host's own rule (a disagreement raises).
-    With and without it: the oracle's distances (round counts may differ by
-    the timing of in-round updates, as between any two solves); R-MAT f32
-    with heuristic, learned and all-edge schedules, ER u32, BFS."""
-    src, dst, w = S.rmat(17, 16, seed=23, threads=8)
-    V = 1 << 17
-    sources = [0] + list(S.pick_sources(src, V, 233, 2))
-    with S.DeviceGraph(src, dst, w, V) as g:
-        g.set_param("plain_below_edges", 0)  # exercise the round engine
-        g.set_param("tail_edges", 1 << 15)
-        for s in sources:
-            for sched in ("heuristic", "learned", "all"):
-                print(f"queued dense rounds: rmat17 source {s} {sched}", flush=True)
-                runs = []
-                for spec in (1, 0):
-                    g.set_param("spec_dense", spec)
-                    d, st = g.solve(int(s), sched)
-                    runs.append((d, st))
-                (d1, s1), (d0, s0) = runs
-                assert np.array_equal(bits(d1), bits(d0))
-                assert np.array_equal(bits(d1), bits(oracle.dijkstra(src, dst, w, V, int(s))))
-                assert s1["verify_passes"] == s0["verify_passes"] == 1
-        g.set_param("spec_dense", 1)
-        vis, lev, _ = g.bfs(int(sources[1]), "heuristic")
-        rvis, rlev, _ = oracle.bfs(src, dst, V, int(sources[1]))
-        assert np.array_equal(lev, rlev)
-    V = 1 << 15
-    src, dst, w = S.er(V, 8, seed=29, weight_mode=1)
-    with S.DeviceGraph(src, dst, w, V) as g:
-        g.set_param("plain_below_edges", 0)
-        print("queued dense rounds: er15 u32", flush=True)
-        d, st = g.solve(3, "heuristic")
-        assert np.array_equal(d, oracle.dijkstra(src, dst, w, V, 3))
-
-
-@pytest.mark.gpu
 @pytest.mark.parametrize("scale,ef,seed,threads", [(12, 16, 3, 64), (16, 16, 0, 1), (18, 8, 11, 7)])
 def test_device_rmat_generator_matches_host(gpu, scale, ef, seed, threads):
     s, d, w = S.rmat(scale, ef, seed=seed, threads=threads)
