@@ -1078,6 +1078,10 @@ __global__ __launch_bounds__(kBlock) void k_round_open(Ctrl *c, int32_t r, int p
 // decided rounds resume (tail_stop).
 // ---------------------------------------------------------------------------
 constexpr int kTailChunk = 2;  // chunk bucket of the tail rounds' relaxation (4 edges per lane)
+// ... and of a run of tail rounds whose first frontier has more out-edges
+// than kTailBigEdges (8 edges per lane, the decided push rounds' grid)
+constexpr int kTailBigChunk = 3;
+constexpr int64_t kTailBigEdges = (int64_t)1 << 22;
 
 struct TailArgs {
     Ctrl *ctrl;
@@ -2141,8 +2145,7 @@ int32_t run_tail(sssp_graph *g, int32_t r, int32_t tick0, int32_t r_cap, sssp_st
     // (measured at R-MAT 25 x 16 and 22 x 32: 2 per CU 1-2 % faster than 4)
     const unsigned grid = (unsigned)std::max(1, g->cu_count * 2);
     const int64_t nb = g->tail_blocks > 0 ? g->tail_blocks : (int64_t)g->cu_count * 2;
-    const unsigned agrid = (unsigned)std::max<int64_t>(8, (nb + 7) / 8 * 8);
-    const int64_t TE = (int64_t)kBlock * kChunkK[kTailChunk];
+    const unsigned agrid_small = (unsigned)std::max<int64_t>(8, (nb + 7) / 8 * 8);
     TailArgs a{};
     a.ctrl = g->ctrl;
     a.mirror = g->ctrl_host_dev;
@@ -2159,6 +2162,10 @@ int32_t run_tail(sssp_graph *g, int32_t r, int32_t tick0, int32_t r_cap, sssp_st
         const int n = (int)std::min<int64_t>(group, (int64_t)r_cap - r);
         if (n <= 0) break;
         const RoundSlot in = h.slot[r & 1];  // the frontier the first queued round consumes
+        const bool big = in.qedges > kTailBigEdges;
+        const int kidx = big ? kTailBigChunk : kTailChunk;
+        const int64_t TE = (int64_t)kBlock * kChunkK[kidx];
+        const unsigned agrid = big ? (unsigned)g->cu_count * 8 : agrid_small;
         if (g->trace_events) SSSP_HIP(hipEventRecord(g->ev_r0, s));
         for (int i = 0; i < n; i++) {
             a.r = r + i;
@@ -2180,7 +2187,7 @@ int32_t run_tail(sssp_graph *g, int32_t r, int32_t tick0, int32_t r_cap, sssp_st
             adv.n = 1;  // the counts are read on the device
             adv.T = 1;
             adv.ntiles = 1;
-            launch_adv_k<WT, DIR_PUSH, SRC_SPARSE, PRED_NONE>(kTailChunk, agrid, s, adv);
+            launch_adv_k<WT, DIR_PUSH, SRC_SPARSE, PRED_NONE>(kidx, agrid, s, adv);
             k_tail_close<<<1, 64, 0, s>>>(a);
             SSSP_LAUNCH_CHECK("k_tail_close");
             a.first = 0;
