@@ -21,6 +21,29 @@ def declared_symbols():
     return sorted(set(re.findall(r"\b(sssp_[a-z0-9_]+)\s*\(", text)))
 
 
+def test_documented_knobs_are_handled():
+    """Every knob the header lists for sssp_set_param / sssp_get_param is a
+    name the C ABI's get (all) and set (all but the read-only ones) compare
+    against, and every name they handle is documented."""
+    text = open(HEADER).read()
+    block = text[text.index("/* Tuning knobs by name"):]
+    block = block[:block.index("*/")]
+    settable_part, ro_part = block.split(";", 1)
+    ro_part = ro_part[:ro_part.index("read-only")]
+    settable = set(re.findall(r'"([a-z_0-9]+)"', settable_part))
+    read_only = set(re.findall(r'"([a-z_0-9]+)"', ro_part))
+    assert len(settable) > 20 and {"tail_edges", "plain_kernel"} <= settable
+    src = open(os.path.join(ROOT, "parallel-task1_amd", "csrc", "capi.cpp")).read()
+    set_body = src[src.index("int sssp_set_param("):src.index("int sssp_get_param(")]
+    get_body = src[src.index("int sssp_get_param("):]
+    get_body = get_body[:get_body.index("\n}\n")]
+    in_set = set(re.findall(r'n == "([a-z_0-9]+)"', set_body))
+    in_get = set(re.findall(r'n == "([a-z_0-9]+)"', get_body))
+    assert settable <= in_set, sorted(settable - in_set)
+    assert (settable | read_only) <= in_get, sorted((settable | read_only) - in_get)
+    assert in_set | in_get <= settable | read_only, sorted((in_set | in_get) - settable - read_only)
+
+
 def test_library_exports_every_declared_symbol(lib):
     names = declared_symbols()
     assert len(names) > 40
