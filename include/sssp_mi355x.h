@@ -186,7 +186,7 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * "collect_budget", "dense_all_frac", "dense_unreached", "dense_kernel",
  * "batch_capture", "decide_spin", "blk_tile_cap", "blk_tile_order",
  * "blk_range_bits", "plain_kernel", "tail_edges", "tail_group", "tail_blocks",
- * "trace_events"; "cu_count",
+ * "push_bits_edges", "trace_events"; "cu_count",
  * "collect_rounds", "multi_passes", "blk_tiles", "blk_wide_chunks" and
  * "plain_blocked" read-only).
  * dense_all_frac (default 0.15, or 0.4 for a graph without hubs: no block of
@@ -222,6 +222,10 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * is empty or has more out-edges than tail_edges again; 0 turns them off.
  * Traces show them as SSSP_MODE_TAIL, with sizes and times only under
  * trace_events (which queues them one at a time).
+ * push_bits_edges (default 0 = off): a decided push round over a frontier
+ * with at least this many out-edges (far pile off) marks the vertices it
+ * lowers in a bitmap (one non-returning OR each) and builds the next queue
+ * from it in one pass, instead of stamping and appending per lowered vertex.
  * delta > 0 turns on the far pile of push rounds: a frontier vertex with
  * d >= (min distance over the frontier and pile) + delta and out-degree >=
  * hub_degree waits in the pile instead of being relaxed; 0 (default) = off.

@@ -45,7 +45,7 @@ void graph_free(sssp_graph *g)
                     g->blk_a, g->blk_b, g->parent, g->ctrl, g->bdist, g->bscratch,
                     g->src_list, g->btasks, g->bcounter, g->bround, g->x_src, g->x_dst,
                     g->x_w, g->xgoff, g->pile[0], g->pile[1], g->farst, g->snap,
-                    g->b_pk, g->b_cbase, g->b_wide, g->b_w, g->b_tile, g->b_app, g->m_rec, g->m_tile, g->mdist,
+                    g->b_pk, g->b_cbase, g->b_wide, g->b_w, g->b_tile, g->b_app, g->fbits, g->m_rec, g->m_tile, g->mdist,
                     g->vperm, g->vinv, g->dout != g->dist ? g->dout : nullptr, g->msrc, g->mflags};
     if (g->batch_exec) (void)hipGraphExecDestroy((hipGraphExec_t)g->batch_exec);
     for (auto &pair : g->mexec)
@@ -186,6 +186,11 @@ void graph_finish(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, con
     g->queue[0] = dmalloc<int32_t>(V + 1);
     g->queue[1] = dmalloc<int32_t>(V + 1);
     g->lb_v = dmalloc<int32_t>(V + 1);
+    {
+        const size_t nfw = 2 * (((size_t)V + 63) / 64) + 2;
+        g->fbits = dmalloc<uint32_t>(nfw);
+        SSSP_HIP(hipMemsetAsync(g->fbits, 0, 4 * nfw, s));
+    }
     g->lb_start = dmalloc<int64_t>(V + 1);
     g->lb_scan = dmalloc<int64_t>(V + 2);
     g->lb_tile = dmalloc<int32_t>((size_t)(g->E / kBlock) + 4);
@@ -624,6 +629,9 @@ int sssp_set_param(sssp_graph *g, const char *name, double value)
         } else if (n == "tail_group") {
             SSSP_REQUIRE(value >= 1 && value <= 64, "tail_group must be in [1, 64]");
             g->tail_group = (int)value;
+        } else if (n == "push_bits_edges") {
+            SSSP_REQUIRE(value >= 0 && value <= 9.2e18, "push_bits_edges must be >= 0");
+            g->push_bits_edges = (int64_t)value;
         } else if (n == "tail_blocks") {
             SSSP_REQUIRE(value >= 0 && value <= 65536, "tail_blocks must be in [0, 65536]");
             g->tail_blocks = (int)value;
@@ -663,6 +671,7 @@ int sssp_get_param(const sssp_graph *g, const char *name, double *value)
         else if (n == "tail_edges") *value = (double)g->tail_edges;
         else if (n == "tail_group") *value = g->tail_group;
         else if (n == "tail_blocks") *value = g->tail_blocks;
+        else if (n == "push_bits_edges") *value = (double)g->push_bits_edges;
         else if (n == "multi_passes") *value = g->multi_passes;
         else if (n == "rollout_policy") *value = g->rollout_policy;
         else if (n == "collect_kind") *value = g->collect_kind;

@@ -67,6 +67,8 @@ struct AdvArgs {
     const int64_t *dev_T;      // from the device (near-far compaction)
     const unsigned long long *dev_nt;  // when set, both from one word: count << 32 | total
                                        // (tail rounds)
+    uint32_t *fbits;           // push: when set, lowered vertices are marked here (one bit
+                               // each) instead of appended; k_bits_frontier queues them
 };
 
 constexpr int kStageCap = 512;  // per-wave LDS staging slots for frontier appends
@@ -419,12 +421,80 @@ __global__ __launch_bounds__(kBlock) void k_advance(AdvArgs a)
             for (int k = 0; k < K; k++)
                 if (lowk[k]) lower_dist(a.dist, vk[k], ck[k]);
         }
-        append_frontier_n<DIR, K>(lowk, vk, a.row_ptr, a.stamp, a.tick_cur, acc.qedges, wbuf,
-                                  wcnt, a.qnext, &a.slot->qlen);
+        if (DIR == DIR_PUSH && a.fbits) {
+            // one non-returning OR per lowered vertex: no stamp read, no
+            // degree loads, no returning exchange here (k_bits_frontier)
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (lowk[k])
+                    __hip_atomic_fetch_or(&a.fbits[vk[k] >> 5], 1u << (vk[k] & 31),
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            append_frontier_n<DIR, K>(lowk, vk, a.row_ptr, a.stamp, a.tick_cur, acc.qedges,
+                                      wbuf, wcnt, a.qnext, &a.slot->qlen);
+        }
         __syncthreads();
     }
     stage_flush_block<kBlock / 64>(s_stage, wcnt, a.qnext, &a.slot->qlen);
     flush_acc(acc, a.slot);
+}
+
+// The next frontier of a push round that marked its lowered vertices in a
+// bitmap (AdvArgs::fbits): every vertex with out-edges whose bit is set joins
+// the queue (in vertex order) with the round's stamp, and its bit is
+// cleared. A wave takes 8 chunks of 64 vertices per step (two bitmap words
+// each): the 8 words, then the row offsets of the set bits, are loaded
+// together; chunks without a set bit cost only their word load.
+__global__ __launch_bounds__(kBlock) void k_bits_frontier(uint32_t *fbits, int64_t V,
+                                                          const int64_t *row_ptr, int32_t *stamp,
+                                                          int32_t tick, int32_t *q,
+                                                          RoundSlot *slot)
+{
+    constexpr int J = 8;
+    __shared__ int32_t s_stage[kBlock / 64][kStageCap];
+    const int tid = threadIdx.x, lane = __lane_id();
+    int32_t *wbuf = s_stage[tid >> 6];
+    int wcnt = 0;
+    RoundAcc acc;
+    const int64_t nch = (V + 63) >> 6;  // the bitmap holds 2 * nch words
+    const int64_t ngroups = (nch + J - 1) / J;
+    const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+    for (int64_t gi = (int64_t)blockIdx.x * (kBlock / 64) + (tid >> 6); gi < ngroups; gi += nw) {
+        uint32_t wd[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int64_t c = gi * J + j;
+            wd[j] = c < nch ? fbits[c * 2 + (lane >> 5)] : 0u;
+        }
+        bool set[J];
+        int64_t r0[J], r1[J];
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int64_t v = (gi * J + j) * 64 + lane;
+            set[j] = v < V && ((wd[j] >> (lane & 31)) & 1u);
+            r0[j] = r1[j] = 0;
+            if (set[j]) {
+                r0[j] = row_ptr[v];
+                r1[j] = row_ptr[v + 1];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int64_t c = gi * J + j;
+            const int32_t v = (int32_t)(c * 64 + lane);
+            const int64_t deg = r1[j] - r0[j];
+            const bool app = set[j] && deg > 0;
+            if (app) {
+                stamp[v] = tick;
+                acc.qedges += deg;
+            }
+            stage_push(app, v, wbuf, wcnt);
+            if (wcnt > kStageCap - 64) stage_flush(wbuf, wcnt, q, &slot->qlen);
+            if ((lane & 31) == 0 && wd[j] != 0u) fbits[c * 2 + (lane >> 5)] = 0u;
+        }
+    }
+    stage_flush_block<kBlock / 64>(s_stage, wcnt, q, &slot->qlen);
+    flush_acc(acc, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -2008,6 +2078,10 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
             }
         }
         lb_prepare(g, q, nbound, piled ? &g->ctrl->lb_n : nullptr, kidx, rule, p ^ 1);
+        // a large frontier marks its lowered vertices in the bitmap
+        const bool bits = g->fbits && g->push_bits_edges > 0 && !piled &&
+                          ebound >= g->push_bits_edges;
+        a.fbits = bits ? g->fbits : nullptr;
         a.ent_v = g->lb_v;
         a.ent_start = g->lb_start;
         a.ent_scan = g->lb_scan;
@@ -2019,6 +2093,11 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
         a.adj = g->el_dst;
         a.wts = g->el_w;
         launch_advance<WT>(g, Kind::SparsePush, kidx, a);
+        if (bits) {
+            k_bits_frontier<<<(unsigned)g->cu_count * 8, kBlock, 0, s>>>(
+                g->fbits, g->V, g->row_ptr, g->stamp, a.tick_cur, a.qnext, a.slot);
+            SSSP_LAUNCH_CHECK("k_bits_frontier");
+        }
         return piled;
     }
     case SSSP_MODE_PULL: {

@@ -270,6 +270,10 @@ struct sssp_graph {
     int tail_group = 4;    // tail rounds queued per host check
     int tail_blocks = 0;   // workgroups of a tail round's relaxation (0 = 2 per CU)
     bool tail_armed = false;  // this solve's frontier has had more than tail_edges out-edges
+    uint32_t *fbits = nullptr;  // push rounds' lowered-vertex bitmap (2 * ceil(V / 64) words, zero
+                                // between rounds)
+    int64_t push_bits_edges = 0;  // decided push rounds over at least this many frontier
+                                  // out-edges mark lowered vertices in fbits (0 = never)
     bool last_round_dense = false;  // the last round ran dense (traces)
     int rollout_policy = 0;        // timing collection: rounds after the tried one follow 0 rule, 1 network
     int collect_kind = 0;          // timing collection of 0 SSSP, 1 BFS rounds

@@ -257,6 +257,43 @@ def test_tail_rounds_bit_exact(gpu, oracle, tail_edges, group):
 
 
 @pytest.mark.gpu
+def test_push_rounds_with_frontier_bitmap(gpu, oracle):
+    """Push rounds that mark their lowered vertices in a bitmap and queue them
+    in one pass (push_bits_edges) land on the oracle's fixpoint: f32 R-MAT with
+    the push-only, heuristic and learned schedules (tail rounds on and off),
+    u32 ER, BFS levels."""
+    src, dst, w = S.rmat(16, 16, seed=31, threads=8)
+    V = 1 << 16
+    sources = [0] + list(S.pick_sources(src, V, 311, 3))
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)  # exercise the round engine
+        g.set_param("push_bits_edges", 1)    # every push round
+        for tail in (0, 1 << 14):
+            g.set_param("tail_edges", tail)
+            for s in sources:
+                for sched in ("push", "heuristic", "learned"):
+                    print(f"bitmap push rounds: rmat16 source {s} {sched} tail {tail}", flush=True)
+                    d, st = g.solve(int(s), sched)
+                    assert np.array_equal(bits(d), bits(oracle.dijkstra(src, dst, w, V, int(s)))), \
+                        (s, sched, tail)
+        ok, counts, par = g.certify(int(sources[-1]))
+        assert ok, counts
+        print("bitmap push rounds: rmat16 BFS", flush=True)
+        vis, lev, _ = g.bfs(int(sources[1]), "push")
+        rvis, rlev, _ = oracle.bfs(src, dst, V, int(sources[1]))
+        assert np.array_equal(lev, rlev)
+    V = 1 << 15
+    src, dst, w = S.er(V, 8, seed=37, weight_mode=1)
+    with S.DeviceGraph(src, dst, w, V) as g:
+        g.set_param("plain_below_edges", 0)
+        g.set_param("push_bits_edges", 1)
+        print("bitmap push rounds: er15 u32", flush=True)
+        for sched in ("push", "heuristic"):
+            d, st = g.solve(9, sched)
+            assert np.array_equal(d, oracle.dijkstra(src, dst, w, V, 9)), sched
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("scale,ef,seed,threads", [(12, 16, 3, 64), (16, 16, 0, 1), (18, 8, 11, 7)])
 def test_device_rmat_generator_matches_host(gpu, scale, ef, seed, threads):
     s, d, w = S.rmat(scale, ef, seed=seed, threads=threads)
