@@ -196,7 +196,8 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * of a fixed vertex sample unreached, runs dense instead of appending per
  * edge; the fixpoint check always does. dense_kernel 0 (default) runs dense
  * rounds over the destination-blocked edge list (LDS minima per block, the
- * lowered vertices appended by the same kernel); 1 runs the plain pass, then
+ * lowered vertices marked by the same kernel and queued when a push round
+ * needs them); 1 runs the plain pass, then
  * compacts the lowered vertices against a snapshot into the next frontier.
  * rollout_policy and collect_kind steer sssp_collect_timings: the rounds
  * after the tried one follow the rule (0) or the attached network (1); the

@@ -2012,7 +2012,7 @@ bool run_round(sssp_graph *g, int mode, int kidx, int32_t r, int32_t tick0, cons
             // lowers most vertices it meets -- relaxing every reached source
             // without per-edge appends costs less than the edge loop's
             // appends: over the destination-blocked list (LDS minima, no
-            // global atomics, the frontier written by the same kernel), or
+            // global atomics, the frontier marked by the same kernel), or
             // the plain pass followed by a compaction of the lowered vertices
             // against a snapshot
             g->last_round_dense = true;
