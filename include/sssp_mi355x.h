@@ -243,8 +243,8 @@ int sssp_certify(sssp_graph *g, int32_t source, int32_t *parent_out, int64_t *co
  * per check. "multi_passes" (read-only) = passes the last policy-3 batch ran.
  * blk_tile_cap / blk_tile_order rebuild the tile table of the dense rounds'
  * destination-blocked list: at most blk_tile_cap edge slots per tile (a
- * multiple of 256; default 2^17, doubled while 4096 tiles of the cap fit in
- * E) and the dispatch order 0 = smallest tiles first (default), 1 = by block,
+ * multiple of 256; default 2^17, doubled while more than 4096 tiles of the
+ * cap fit in E) and the dispatch order 0 = smallest tiles first (default), 1 = by block,
  * 2 = largest first, 3 = by descending block. blk_range_bits > 0 (18-31)
  * also cuts every block's tiles where their sources cross multiples of
  * 2^blk_range_bits and dispatches the tiles range by range, so that the

@@ -1078,15 +1078,16 @@ void build_layouts(sssp_graph *g, const int32_t *d_src, const int32_t *d_dst, co
         for (int32_t b = 0; b < nblk; b++)
             bo[(size_t)b + 1] = bo[(size_t)b] +
                                 (bp[(size_t)b + 1] - bp[(size_t)b] + kBlkChunk - 1) / kBlkChunk * kBlkChunk;
-        // Tile cap: 2^17 edges below 2^29 edges, doubled while kBlkTileCount
-        // tiles of the cap would not exceed the graph -- the 2.08 G-edge R-MAT
-        // graph ran 50.9 GTEPS with 2^19-edge tiles, 46.7 with 2^17; with
-        // degree-ordered slots R-MAT 24 x 16 ran 11.3 / 11.4 ms (learned) at
-        // 2^17 / 2^18 and R-MAT 25 x 16 (2^29 edges) 19.6 / 19.2 ms, its
-        // dense round 1.58 / 1.43 ms. A power of two, so every tile holds
+        // Tile cap: 2^17 edges up to 2^29 edges, doubled while more than
+        // kBlkTileCount tiles of the cap fit in the graph -- the 2.08 G-edge
+        // R-MAT graph ran 50.9 GTEPS with 2^19-edge tiles, 46.7 with 2^17;
+        // R-MAT 25 x 16 (2^29 edges), learned / plain ms, with the dense
+        // round's stream loaded a step ahead: 14.9 / 22.3 at 2^17, 15.6 /
+        // 23.3 at 2^18, 17.0 / 26.3 at 2^19 (before that change 2^18 was the
+        // faster: 19.2 against 19.6 ms). A power of two, so every tile holds
         // whole chunks.
         int64_t tcap = kBlkTileEdges;
-        while (tcap * kBlkTileCount <= E) tcap <<= 1;
+        while (tcap * kBlkTileCount < E) tcap <<= 1;
         g->b_off_host = bo;
         std::vector<BlkTile> tiles = make_blk_tiles(bo, tcap, 0, BlkRanges{});
         g->blk_tile_cap = tcap;

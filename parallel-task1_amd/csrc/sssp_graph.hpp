@@ -73,7 +73,7 @@ struct Ctrl {
 // bits) and its source as the delta to the previous edge of the chunk, and
 // its weight; each chunk keeps its first source whole. A tile is a run of
 // at most kBlkTileEdges slots of one block, whole chunks (the cap doubles
-// while the graph holds kBlkTileCount tiles' worth of edges or more);
+// while the graph holds more than kBlkTileCount tiles' worth of edges);
 // `split` = the block has more than one tile.
 constexpr int kBlkBits = 14;
 constexpr int kBlkThreads = 1024;  // threads of a blocked-pass workgroup

@@ -238,7 +238,7 @@ def test_c5_rmat25_learned_heuristic_plain(gpu, oracle):
     OpenMP threads). The learned schedule must beat the plain passes over the
     edge list by the 1.5x target; against the plain passes over the
     destination-blocked list (the default plain kernel at this size, which
-    shares the dense rounds' layout and LDS minima) it measures 1.49-1.59x,
+    shares the dense rounds' layout and LDS minima) it measures 1.49-1.6x,
     and the test guards it at 1.25x."""
     g, (src, dst, w) = S.DeviceGraph.rmat(25, 16, seed=6, threads=64, download_edges=True)
     with g:
