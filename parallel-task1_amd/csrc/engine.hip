@@ -1182,7 +1182,9 @@ __device__ __forceinline__ bool tail_runs(const TailArgs &a)
 // The entry list of a tail round, built without a separate scan: each wave
 // takes 64 frontier vertices, scans their out-degrees across its lanes and
 // reserves its entries and their edges with one 64-bit atomic (count << 32 |
-// edges), so entries stay in edge order across waves. Writes the entries
+// edges), so entries stay in edge order across waves; neither half carries
+// into the other, as the frontier holds distinct vertices (count < V < 2^31)
+// and their out-edges number at most E < 2^31. Writes the entries
 // (vertex, row start, exclusive edge offset) and, for each tile of TE edges,
 // the entry holding its first edge -- the load-balancing lists k_advance
 // reads (k_lb_reduce / k_lb_write build the same for decided rounds).
