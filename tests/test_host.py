@@ -217,3 +217,14 @@ def test_edge_fingerprint_tracks_edits_not_order():
     i, j = 0, 1 + int(np.flatnonzero(w[1:] != w[0])[0])
     w3[i], w3[j] = w[j], w[i]
     assert S.edge_fingerprint(s, d, w3) != f0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: asked for several it exits with status 2 and
+    says why, before it touches a device or prints a result line."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and "--gpus 2" in r.stderr
+    assert not r.stdout.strip()
